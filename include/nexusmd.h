@@ -1,0 +1,190 @@
+/*
+ * nexusmd.h -- C-ABI of the MI355X force-and-integrate engine for NexusMD.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes.  No C++
+ * types, no exceptions and no torch types cross this boundary.  The library
+ * (libnexusmd_hip.so) is built for gfx950 only.
+ *
+ * Reference interfaces replaced (NgFEP/NexusMD, paths from the repo root,
+ * "core/" = "NexusMD/core/"):
+ *   - BaseLine::Engine (core/baseline/headers/Engine.h:25-93) and the loop in
+ *     Engine::RunSimulation (core/baseline/src/Engine.cpp:162-188);
+ *   - Forces::AddHBond/AddHAngle/AddPTorsion/AddNonBondElectroPME
+ *     (core/baseline/headers/Forces/Forces.h:21-24);
+ *   - VerletIntegration::Advance (core/baseline/headers/VerletIntegration.h:17);
+ *   - KineticEnergy::calculateKineticEnergy (core/baseline/headers/KineticEnergy.h:12);
+ *   - obtainNeighborList (core/baseline/headers/Forces/NeighborList.h:62-69).
+ * Units are the reference's: nm, ps, amu, kJ/mol, e.
+ *
+ * Threading: one context per host thread; calls on one context serialise on
+ * its HIP stream.  A server handle is likewise single-threaded.
+ */
+#ifndef NEXUSMD_H
+#define NEXUSMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int nx_status;
+enum {
+    NX_OK = 0,
+    NX_ERR_ARG = 1,       /* invalid argument (index out of range, bad size, NULL) */
+    NX_ERR_HIP = 2,       /* HIP runtime error */
+    NX_ERR_CAPACITY = 3,  /* a device list overflowed its capacity */
+    NX_ERR_STATE = 4,     /* call out of order (e.g. no topology yet) */
+    NX_ERR_NOMEM = 5,     /* device allocation failed */
+    NX_ERR_NUMERIC = 6    /* non-finite energy or position detected */
+};
+
+enum { NX_PRECISION_FP64 = 0, NX_PRECISION_MIXED = 1 };
+
+/* Enable bits, one per TaskDispatcher::Enable*Force (core/dispatcher/headers/TaskDispatcher.h:22-25) */
+enum { NX_FORCE_BOND = 1, NX_FORCE_ANGLE = 2, NX_FORCE_TORSION = 4, NX_FORCE_NONBONDED = 8, NX_FORCE_ALL = 15 };
+
+/* Parameter records; same fields and meaning as BondParams / AngleParams /
+ * PTorsionParams (core/common/headers/SystemXMLParser.h:10-27). */
+typedef struct { int p1, p2; double d, k; } nx_bond;
+typedef struct { int p1, p2, p3; double a, k; } nx_angle;
+typedef struct { int p1, p2, p3, p4; double k, phase; int periodicity; } nx_torsion;
+
+/* Nonbonded options.  cutoff and ewald_tolerance follow NonbondedParams
+ * (SystemXMLParser.h:34-39); alpha = sqrt(-ln(2 tol)) / cutoff
+ * (core/baseline/src/Forces/NonbondedForce.cpp:297-300). */
+typedef struct {
+    double cutoff;
+    double ewald_tolerance;
+    int grid[3];            /* PME grid; {0,0,0} = reference rule (next pow2 >= 10*ceil(L)) */
+    int pme_order;          /* must be 5 (NonbondedForce.cpp:474) */
+    int use_switch;         /* switching function (NonbondedForce.cpp:553-557, applied per pair) */
+    double switch_distance;
+    double skin;            /* Verlet buffer in nm; 0 = rebuild every step as the reference does */
+} nx_nonbonded_opts;
+
+typedef struct {
+    int n_atoms;
+    const double* mass;                               /* [n_atoms] amu */
+    int n_bonds;    const nx_bond* bonds;
+    int n_angles;   const nx_angle* angles;
+    int n_torsions; const nx_torsion* torsions;
+    const double* charge;                             /* [n_atoms] e */
+    const double* sigma;                              /* [n_atoms] nm */
+    const double* epsilon;                            /* [n_atoms] kJ/mol */
+    const int* excl_offsets;                          /* CSR [n_atoms+1] or NULL: derive 1-2/1-3/1-4 from bonds */
+    const int* excl_indices;                          /*  (Exclusions.cpp:7-44, bondCutoff = 3) */
+    nx_nonbonded_opts nb;
+} nx_topology;
+
+/* Per-term energies, kJ/mol.  reference_pe is what the reference's
+ * TotalEnergy report prints: with the nonbonded force enabled, the
+ * reciprocal energy is assigned over the bonded sum (NonbondedForce.cpp:387). */
+typedef struct {
+    double bond, angle, torsion;
+    double coulomb_direct, lj, recip, self, exclusion;
+    double potential;       /* physical total of the enabled terms */
+    double reference_pe;
+    double kinetic;         /* half-step-shifted KE (KineticEnergy.cpp:21-41) */
+} nx_energies;
+
+typedef struct nx_ctx nx_ctx;
+
+/* Context lifetime.  Replaces Engine::Engine (Engine.cpp:19-50). */
+nx_status nx_create(int device, int precision, nx_ctx** out);
+nx_status nx_destroy(nx_ctx* ctx);
+const char* nx_last_error(const nx_ctx* ctx);
+const char* nx_status_string(nx_status s);
+
+/* Topology upload (Engine::InitializeSimulationParameters, Engine.cpp:53-73,
+ * plus Exclusions::createExclusions, Engine.cpp:171). */
+nx_status nx_set_topology(nx_ctx* ctx, const nx_topology* top);
+
+/* Box from lower/upper bounds (BoxInfo, core/common/headers/PeriodicBoundaryCondition.h:11-19). */
+nx_status nx_set_box(nx_ctx* ctx, const double lb[3], const double ub[3]);
+/* The reference's rule: bounding box of the positions widened by 1% per side
+ * (PeriodicBoundaryCondition::extractBoxBoundaries, PeriodicBoundaryCondition.cpp:11-88). */
+nx_status nx_box_from_positions(int n, const double* pos, double lb[3], double ub[3]);
+
+/* State in/out, AoS double[n][3] as Coords3D.  vel may be NULL (zeros, as
+ * Initializer::InitializeForcesAndVelocities, Initializer.cpp:18-24). */
+nx_status nx_set_state(nx_ctx* ctx, const double* pos, const double* vel);
+nx_status nx_get_state(nx_ctx* ctx, double* pos, double* vel, double* forces);
+
+/* Forces and energies at the current positions (Engine::CalculateForces,
+ * Engine.cpp:75-95).  forces may be NULL; e may be NULL. */
+nx_status nx_compute_forces(nx_ctx* ctx, int force_mask, double* forces, nx_energies* e);
+
+/* nsteps x [forces -> leapfrog] on the device (the loop of
+ * Engine::RunSimulation without reports; VerletIntegration::Advance,
+ * VerletIntegration.cpp:49-89). */
+nx_status nx_step(nx_ctx* ctx, int nsteps, double dt, int force_mask);
+
+/* One step whose report matches Engine::Report (Engine.cpp:117-158): forces
+ * and energies at x(t), kinetic energy with the half-step shift, and x(t),
+ * v(t-dt/2), F(t) copied out before the leapfrog update.  Outputs may be NULL. */
+nx_status nx_step_report(nx_ctx* ctx, double dt, int force_mask, double* pos, double* vel, double* forces,
+                         nx_energies* e);
+
+/* The explicit pair set of the current positions: (i, j), i > j, r^2 < rc^2,
+ * not excluded -- the set obtainNeighborList emits (NeighborList.cpp:129-154).
+ * Writes up to cap pairs; *count receives the full count. */
+nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count);
+
+/* Kernel timing of the last nx_step/nx_compute_forces call, per kernel
+ * family, in milliseconds per launch (average over launches in the call).
+ * names: "pair", "pme_spread", "fft", "pme_gather", "bonded", "integrate",
+ * "nlist", "exclusion".  Timing is only collected when enabled. */
+nx_status nx_enable_timing(nx_ctx* ctx, int on);
+nx_status nx_kernel_time(nx_ctx* ctx, const char* name, double* ms_per_launch, int64_t* launches);
+
+/* Counters of the last call: pairs evaluated by the pair kernel (cluster
+ * tiles x 64), pair-list entries, neighbour-list rebuilds. */
+typedef struct {
+    int64_t n_slots, n_clusters, n_super, list_entries, tiles, rebuilds, steps;
+    int grid[3];
+    double alpha;
+} nx_stats;
+nx_status nx_get_stats(nx_ctx* ctx, nx_stats* s);
+
+/* Standalone 3D FFT through the engine's Stockham kernels, for verification:
+ * real input [nx][ny][nz] (row-major) -> half spectrum [nx][ny][nz/2+1]
+ * (interleaved complex, unnormalised, sign -1), and the inverse (sign +1)
+ * back to real.  precision as nx_create. */
+nx_status nx_fft3d_r2c(int device, int precision, int nx, int ny, int nz, const double* in, double* out);
+nx_status nx_fft3d_c2r(int device, int precision, int nx, int ny, int nz, const double* in, double* out);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous batching of many independent small systems on one GPU (no
+ * counterpart in the reference).  Requests are admitted while their device
+ * footprint fits under hbm_budget_bytes, advance together in batched
+ * launches, and retire as they finish; new requests join at the next step
+ * boundary. */
+typedef struct nx_server nx_server;
+typedef struct {
+    int64_t submitted, admitted, finished, failed, active;
+    double p50_step_ms, p99_step_ms;      /* per-request step latency */
+    double aggregate_ns_per_day;          /* sum of simulated time / wall time */
+    double wall_s;
+    int64_t batched_steps;                /* batched device steps executed */
+    int64_t bytes_in_use, bytes_budget;
+} nx_server_stats;
+
+nx_status nx_server_create(int device, int precision, int64_t hbm_budget_bytes, int max_active, nx_server** out);
+nx_status nx_server_destroy(nx_server* s);
+const char* nx_server_last_error(const nx_server* s);
+/* Copies topology and state; box from lb/ub.  *req receives the request id. */
+nx_status nx_submit(nx_server* s, const nx_topology* top, const double lb[3], const double ub[3],
+                    const double* pos, const double* vel, int nsteps, double dt, int64_t* req);
+/* Runs batched steps until every active request has advanced `steps` more
+ * steps or finished (steps <= 0: until all admitted work is done). */
+nx_status nx_server_run(nx_server* s, int steps);
+/* status: 0 queued, 1 running, 2 done, 3 failed */
+nx_status nx_poll(nx_server* s, int64_t req, int* status, int* steps_done);
+nx_status nx_fetch(nx_server* s, int64_t req, double* pos, double* vel, nx_energies* e);
+nx_status nx_server_get_stats(nx_server* s, nx_server_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,841 @@
+// engine.hip -- host orchestration of the device engine and the C-ABI.
+//
+// Replaces BaseLine::Engine (core/baseline/src/Engine.cpp:19-188): topology
+// and box set-up once, then per step  [neighbour-list rebuild if flagged] ->
+// pack -> bonded -> exclusion correction -> pair -> PME -> (report-step
+// kinetic energy) -> leapfrog, all enqueued on one HIP stream; the
+// report-free step is captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nexusmd.h"
+#include "nx_device.h"
+#include "nx_engine.h"
+#include "nx_host.h"
+
+namespace nx {
+
+#define HIPCHK(x)                                                                                        \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess)                                                                            \
+            throw ::nx::HipError(e_ == hipErrorOutOfMemory ? NX_ERR_NOMEM : NX_ERR_HIP,                  \
+                                 std::string(#x) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+template <class T>
+static void dalloc(T** p, size_t n, std::vector<void*>& owned) {
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+    owned.push_back(*p);
+}
+
+template <class T>
+static void upload(T* d, const std::vector<T>& h, hipStream_t st) {
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// ------------------------------------------------------------- host setup
+
+// 1-2, 1-3 and 1-4 partners (graph distance <= 3 over the bond graph), the
+// set Exclusions::createExclusions builds with bondCutoff = 3
+// (core/baseline/src/Forces/Exclusions.cpp:7-44).
+void exclusions_from_bonds(int n, const std::vector<nx_bond>& bonds, std::vector<int>& off, std::vector<int>& idx) {
+    std::vector<std::vector<int>> adj(n);
+    for (const auto& b : bonds) {
+        if (b.p1 < 0 || b.p2 < 0 || b.p1 >= n || b.p2 >= n)
+            throw HipError(NX_ERR_ARG, "createExclusions: Illegal particle index in list of bonds");
+        adj[b.p1].push_back(b.p2);
+        adj[b.p2].push_back(b.p1);
+    }
+    off.assign(n + 1, 0);
+    idx.clear();
+    std::vector<int> mark(n, -1), frontier, nxt, found;
+    for (int i = 0; i < n; ++i) {
+        found.clear();
+        frontier.assign(1, i);
+        mark[i] = i;
+        for (int depth = 0; depth < 3; ++depth) {
+            nxt.clear();
+            for (int u : frontier)
+                for (int v : adj[u])
+                    if (mark[v] != i) {
+                        mark[v] = i;
+                        nxt.push_back(v);
+                        found.push_back(v);
+                    }
+            frontier.swap(nxt);
+        }
+        std::sort(found.begin(), found.end());
+        idx.insert(idx.end(), found.begin(), found.end());
+        off[i + 1] = (int)idx.size();
+    }
+}
+
+// |b(m)|^2 of the order-5 B-spline by the O(K^2) DFT and neighbour-average
+// fix-up of NonbondedForce::computeBSplineParameters (NonbondedForce.cpp:77-142).
+static void bspline_moduli(int K, std::vector<double>& mod) {
+    const int order = 5;
+    double data[5] = {0, 0, 0, 0, 0};
+    data[order - 1] = 0.0;
+    data[1] = 0.0;
+    data[0] = 1.0;
+    for (int i = 3; i < order; ++i) {
+        double div = 1.0 / (i - 1);
+        data[i - 1] = 0.0;
+        for (int j = 1; j < (i - 1); ++j) data[i - j - 1] = div * (j * data[i - j - 2] + (i - j) * data[i - j - 1]);
+        data[0] = div * data[0];
+    }
+    double div = 1.0 / (order - 1);
+    data[order - 1] = div * data[order - 2];
+    for (int i = 1; i < (order - 1); ++i)
+        data[order - i - 1] = div * (i * data[order - i - 2] + (order - i) * data[order - i - 1]);
+    data[0] = div * data[0];
+    std::vector<double> bd(std::max(K, order + 1), 0.0);
+    for (int i = 1; i <= order; ++i) bd[i] = data[i - 1];
+    mod.assign(K, 0.0);
+    for (int i = 0; i < K; ++i) {
+        double sc = 0.0, ss = 0.0;
+        for (int j = 0; j < K; ++j) {
+            double arg = (2 * kPi * i * j) / K;
+            sc += bd[j] * cos(arg);
+            ss += bd[j] * sin(arg);
+        }
+        mod[i] = sc * sc + ss * ss;
+    }
+    for (int i = 0; i < K; ++i)
+        if (mod[i] < 1e-7) mod[i] = (mod[(i - 1 + K) % K] + mod[(i + 1) % K]) / 2.0;
+}
+
+static bool fft_size_ok(int K) {
+    if (K < 2) return false;
+    int m = K;
+    for (int p : {2, 3, 5})
+        while (m % p == 0) m /= p;
+    return m == 1;
+}
+
+// PME grid rule Q2 (NonbondedForce.cpp:33-48).
+static int grid_rule(double L) {
+    int m = (int)ceil(L) * 10, p = 1;
+    while (p < m) p *= 2;
+    return p;
+}
+
+// ----------------------------------------------------------------- engine
+
+template <class P>
+class Engine : public EngineBase {
+   public:
+    using real = typename P::real;
+    using real2 = typename P::real2;
+    using real4 = typename P::real4;
+    using facc = typename P::facc;
+
+    Engine(int device) : device_(device) {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    }
+    ~Engine() override {
+        hipSetDevice(device_);
+        release();
+        if (B.stream) hipStreamDestroy(B.stream);
+        for (auto& e : evpool_) hipEventDestroy(e);
+    }
+
+    void release() {
+        if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+        if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+        for (void* p : owned_) hipFree(p);
+        owned_.clear();
+        Batch<P> fresh;
+        fresh.stream = B.stream;
+        fresh.timing = B.timing;
+        B = fresh;
+    }
+
+    // Lay out and upload a batch of systems.
+    void setup(const std::vector<HostSystem*>& systems) override {
+        hipSetDevice(device_);
+        release();
+        hs_ = systems;
+        const int ns = (int)systems.size();
+        B.nsys = ns;
+        B.sys.assign(ns, SysDev{});
+        long long atom_off = 0, slot_off = 0, col_off = 0, sup_off = 0, list_off = 0, emask_off = 0;
+        long long rg_off = 0, cg_off = 0, tw_off = 0, excl_off_total = 0;
+        int bond_off = 0, angle_off = 0, tors_off = 0;
+        fft_maxK_ = 0;
+        for (int s = 0; s < ns; ++s) {
+            HostSystem& h = *systems[s];
+            SysDev& d = B.sys[s];
+            const int n = h.n;
+            for (int k = 0; k < 3; ++k) {
+                d.lb[k] = h.lb[k];
+                d.ub[k] = h.ub[k];
+                d.L[k] = fabs(h.ub[k] - h.lb[k]);
+                d.invL[k] = 1 / d.L[k];
+            }
+            const double rc = h.nb.cutoff;
+            d.rc = rc;
+            d.rc2 = rc * rc;
+            const double skin = std::max(0.0, h.nb.skin);
+            d.rlist = rc + skin;
+            d.skin_half2 = skin > 0 ? 0.25 * skin * skin : -1.0;
+            d.alpha = (1.0 / rc) * sqrt(-log(2.0 * h.nb.ewald_tolerance));
+            if (h.nb.use_switch && h.nb.switch_distance < rc) {
+                d.sw_on = h.nb.switch_distance;
+                d.sw_inv = 1.0 / (rc - h.nb.switch_distance);
+            } else {
+                d.sw_on = -1.0;
+                d.sw_inv = 0.0;
+            }
+            d.n_atoms = n;
+            d.atom_off = (int)atom_off;
+            // columns of ~8/rho^(1/3) nm so that a cluster of 8 is roughly cubic
+            const double V = d.L[0] * d.L[1] * d.L[2];
+            const double rho = std::max(1e-3, n / V);
+            double cw = cbrt(8.0 / rho);
+            int ncx = std::max(1, (int)floor(d.L[0] / cw));
+            int ncy = std::max(1, (int)floor(d.L[1] / cw));
+            while ((double)n / ((double)ncx * ncy) > 700.0) { ncx += 1; ncy += 1; }
+            d.ncx = ncx;
+            d.ncy = ncy;
+            d.colw[0] = d.L[0] / ncx;
+            d.colw[1] = d.L[1] / ncy;
+            d.col_off = (int)col_off;
+            d.col_cap = ncx * ncy;
+            d.slot_off = (int)slot_off;
+            d.slot_cap = ((n + 8 * ncx * ncy + 63) / 64) * 64;
+            d.super_off = (int)sup_off;
+            d.super_cap = (n + 63) / 64 + ncx * ncy;
+            // pair-list capacity from density: clusters within the column neighbourhood
+            const double w = std::max(d.colw[0], d.colw[1]);
+            const double hz = std::min(d.L[2], 64.0 / (rho * d.colw[0] * d.colw[1]));
+            const double r = d.rlist;
+            double ext = (std::min(d.L[0], w + 2 * r + 2 * w)) * (std::min(d.L[1], w + 2 * r + 2 * w)) *
+                         std::min(d.L[2], hz + 2 * r + 0.5);
+            double per_super = 1.5 * (rho / 8.0) * ext * h.list_scale;
+            per_super = std::min(per_super, 1024.0);
+            per_super = std::max(per_super, 16.0);
+            d.list_off = (int)list_off;
+            d.list_cap = (int)std::min<double>(2.0e9, per_super * d.super_cap + 64);
+            d.emask_off = (int)emask_off;
+            d.emask_cap = (int)(std::max<long long>(256, (long long)h.excl_idx.size() / 2 + 64) * h.list_scale);
+            // PME grid
+            int g[3];
+            for (int k = 0; k < 3; ++k) g[k] = h.nb.grid[k] > 0 ? h.nb.grid[k] : grid_rule(d.L[k]);
+            for (int k = 0; k < 3; ++k)
+                if (!fft_size_ok(g[k]) || g[k] < 6)
+                    throw HipError(NX_ERR_ARG, "PME grid dimension " + std::to_string(g[k]) +
+                                                   " must be >= 6 with prime factors 2, 3, 5 only");
+            for (int k = 0; k < 3; ++k) d.grid[k] = g[k];
+            d.rgrid_off = rg_off;
+            d.cgrid_off = cg_off;
+            for (int k = 0; k < 3; ++k) {
+                d.tw_off[k] = (int)tw_off;
+                tw_off += g[k];
+                fft_maxK_ = std::max(fft_maxK_, g[k]);
+            }
+            rg_off += (long long)g[0] * g[1] * g[2];
+            cg_off += (long long)g[0] * g[1] * (g[2] / 2 + 1);
+            d.bond_off = bond_off;
+            d.n_bonds = (int)h.bonds.size();
+            d.angle_off = angle_off;
+            d.n_angles = (int)h.angles.size();
+            d.tors_off = tors_off;
+            d.n_tors = (int)h.torsions.size();
+            bond_off += d.n_bonds;
+            angle_off += d.n_angles;
+            tors_off += d.n_tors;
+            atom_off += n;
+            slot_off += d.slot_cap;
+            col_off += (long long)ncx * ncy;
+            sup_off += d.super_cap;
+            list_off += d.list_cap;
+            emask_off += d.emask_cap;
+            excl_off_total += (long long)h.excl_idx.size();
+        }
+        if (slot_off > (1 << 27) || list_off > 2000000000LL) throw HipError(NX_ERR_ARG, "batch too large");
+        B.n_atoms = (int)atom_off;
+        B.n_slots_cap = (int)slot_off;
+        B.n_cols = (int)col_off;
+        B.n_super_cap = (int)sup_off;
+        B.n_list_cap = list_off;
+        B.n_emask_cap = emask_off;
+        B.rgrid_total = rg_off;
+        B.cgrid_total = cg_off;
+        B.tw_total = tw_off;
+        B.n_bonds = bond_off;
+        B.n_angles = angle_off;
+        B.n_tors = tors_off;
+        B.pme_any = 1;
+        allocate();
+        upload_all();
+    }
+
+    void allocate() {
+        auto& o = owned_;
+        dalloc(&B.d_sys, B.nsys, o);
+        dalloc(&B.d_cnt, B.nsys, o);
+        const size_t A = B.n_atoms, S = B.n_slots_cap;
+        for (double** p : {&B.X, &B.Y, &B.Z, &B.VX, &B.VY, &B.VZ, &B.invm, &B.mass, &B.q, &B.sig, &B.eps, &B.XB, &B.YB,
+                           &B.ZB})
+            dalloc(p, A, o);
+        dalloc(&B.atom_sys, A, o);
+        dalloc(&B.slot_of_atom, A, o);
+        dalloc(&B.atom_col, A, o);
+        dalloc(&B.FAx, A, o);
+        dalloc(&B.FAy, A, o);
+        dalloc(&B.FAz, A, o);
+        dalloc(&B.slot_sys, S, o);
+        dalloc(&B.atom_of_slot, S, o);
+        dalloc(&B.slot_tmp, S, o);
+        dalloc(&B.pos, S, o);
+        dalloc(&B.lj, S, o);
+        dalloc(&B.FSx, S, o);
+        dalloc(&B.FSy, S, o);
+        dalloc(&B.FSz, S, o);
+        for (int** p : {&B.col_sys, &B.col_count, &B.col_fill, &B.col_slot, &B.col_super}) dalloc(p, B.n_cols, o);
+        dalloc(&B.cl_ctr, S / 8 + 1, o);
+        dalloc(&B.cl_half, S / 8 + 1, o);
+        dalloc(&B.super_sys, B.n_super_cap, o);
+        dalloc(&B.super_item, B.n_super_cap, o);
+        dalloc(&B.list, (size_t)B.n_list_cap, o);
+        dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o);
+        dalloc(&B.rgrid, (size_t)B.rgrid_total, o);
+        dalloc(&B.cgrid, (size_t)B.cgrid_total, o);
+        dalloc(&B.phi, (size_t)B.rgrid_total, o);
+        dalloc(&B.eta, (size_t)B.cgrid_total, o);
+        dalloc(&B.twiddle, (size_t)B.tw_total, o);
+        dalloc(&B.energy, (size_t)B.nsys * E_NTERMS, o);
+        dalloc(&d_count_, 1, o);
+        if (B.n_bonds) { dalloc(&B.bond_idx, B.n_bonds, o); dalloc(&B.bond_par, B.n_bonds, o); }
+        if (B.n_angles) { dalloc(&B.angle_idx, B.n_angles, o); dalloc(&B.angle_par, B.n_angles, o); }
+        if (B.n_tors) { dalloc(&B.tors_idx, B.n_tors, o); dalloc(&B.tors_par, B.n_tors, o); }
+    }
+
+    void upload_all() {
+        hipStream_t st = B.stream;
+        const int ns = B.nsys;
+        std::vector<double> mass, invm, q, sig, eps;
+        std::vector<int> atom_sys, slot_sys, col_sys, super_sys, eoff(1, 0), eidx;
+        std::vector<int2> epairs, bidx;
+        std::vector<double2> bpar, apar;
+        std::vector<int4> aidx, tidx;
+        std::vector<double4> tpar;
+        std::vector<real> eta;
+        std::vector<real2> tw;
+        std::vector<int4> zi, yi, xi;
+        const int maxK = fft_maxK_;
+        const int T = std::max(1, std::min(16, (int)(61440 / (2 * maxK * (int)sizeof(real2)))));
+        B.fft_lines = T;
+        B.fft_tile = T;
+        B.fft_maxK = maxK;
+        for (int s = 0; s < ns; ++s) {
+            HostSystem& h = *hs_[s];
+            SysDev& d = B.sys[s];
+            const int a0 = d.atom_off;
+            for (int i = 0; i < h.n; ++i) {
+                mass.push_back(h.mass[i]);
+                invm.push_back(h.mass[i] == 0.0 ? 0.0 : 1.0 / h.mass[i]);
+                q.push_back(h.q[i]);
+                sig.push_back(h.sig[i]);
+                eps.push_back(h.eps[i]);
+                atom_sys.push_back(s);
+                for (int e = h.excl_off[i]; e < h.excl_off[i + 1]; ++e) {
+                    int j = h.excl_idx[e];
+                    eidx.push_back(a0 + j);
+                    if (j > i) epairs.push_back(make_int2(a0 + i, a0 + j));
+                }
+                eoff.push_back((int)eidx.size());
+            }
+            d.excl_off = 0;
+            d.n_excl = 0;
+            for (int i = 0; i < d.slot_cap; ++i) slot_sys.push_back(s);
+            for (int i = 0; i < d.ncx * d.ncy; ++i) col_sys.push_back(s);
+            for (int i = 0; i < d.super_cap; ++i) super_sys.push_back(s);
+            for (auto& b : h.bonds) { bidx.push_back(make_int2(a0 + b.p1, a0 + b.p2)); bpar.push_back(make_double2(b.d, b.k)); }
+            for (auto& a : h.angles) {
+                aidx.push_back(make_int4(a0 + a.p1, a0 + a.p2, a0 + a.p3, 0));
+                apar.push_back(make_double2(a.a, a.k));
+            }
+            for (auto& t : h.torsions) {
+                tidx.push_back(make_int4(a0 + t.p1, a0 + t.p2, a0 + t.p3, a0 + t.p4));
+                tpar.push_back(make_double4(t.k, t.phase, (double)t.periodicity, 0.0));
+            }
+            // PME tables
+            const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2], Kzh = Kz / 2 + 1;
+            std::vector<double> mod[3];
+            bspline_moduli(Kx, mod[0]);
+            bspline_moduli(Ky, mod[1]);
+            bspline_moduli(Kz, mod[2]);
+            const double factor = kPi * kPi / (d.alpha * d.alpha);
+            const double boxfactor = kPi * d.L[0] * d.L[1] * d.L[2];
+            const int mkx = (Kx + 1) / 2, mky = (Ky + 1) / 2, mkz = (Kz + 1) / 2;
+            for (int kx = 0; kx < Kx; ++kx) {
+                const double mx = (kx < mkx) ? kx : (kx - Kx);
+                const double mhx = mx * d.invL[0];
+                const double bx = boxfactor * mod[0][kx];
+                for (int ky = 0; ky < Ky; ++ky) {
+                    const double my = (ky < mky) ? ky : (ky - Ky);
+                    const double mhy = my * d.invL[1];
+                    const double by = mod[1][ky];
+                    for (int kz = 0; kz < Kzh; ++kz) {
+                        if (kx == 0 && ky == 0 && kz == 0) { eta.push_back((real)0); continue; }
+                        const double mz = (kz < mkz) ? kz : (kz - Kz);
+                        const double mhz = mz * d.invL[2];
+                        const double m2 = mhx * mhx + mhy * mhy + mhz * mhz;
+                        const double bz = mod[2][kz];
+                        const double denom = m2 * bx * by * bz;
+                        eta.push_back((real)(kOne4PiEps0 * exp(-factor * m2) / denom));
+                    }
+                }
+            }
+            const int K3[3] = {Kx, Ky, Kz};
+            for (int a = 0; a < 3; ++a)
+                for (int t = 0; t < K3[a]; ++t) {
+                    const double ang = -2.0 * kPi * (double)t / (double)K3[a];
+                    real2 v;
+                    v.x = (real)cos(ang);
+                    v.y = (real)sin(ang);
+                    tw.push_back(v);
+                }
+            for (int l = 0; l < Kx * Ky; l += T) zi.push_back(make_int4(s, l, std::min(T, Kx * Ky - l), 0));
+            for (int x = 0; x < Kx; ++x)
+                for (int k = 0; k < Kzh; k += T) yi.push_back(make_int4(s, x, k, 0));
+            for (int y = 0; y < Ky; ++y)
+                for (int k = 0; k < Kzh; k += T) xi.push_back(make_int4(s, y, k, 0));
+            self_energy_.resize(ns);
+            double se = 0.0;
+            for (int i = 0; i < h.n; ++i) se -= kOne4PiEps0 * h.q[i] * h.q[i] * d.alpha / kSqrtPi;
+            self_energy_[s] = se;
+        }
+        B.n_excl_pairs = (int)epairs.size();
+        dalloc(&B.excl_off, eoff.size(), owned_);
+        dalloc(&B.excl_idx, std::max<size_t>(1, eidx.size()), owned_);
+        if (B.n_excl_pairs) dalloc(&B.excl_pairs, epairs.size(), owned_);
+        dalloc(&B.fftz_items, zi.size(), owned_);
+        dalloc(&B.ffty_items, yi.size(), owned_);
+        dalloc(&B.fftx_items, xi.size(), owned_);
+        B.n_fftz = (int)zi.size();
+        B.n_ffty = (int)yi.size();
+        B.n_fftx = (int)xi.size();
+        upload(B.d_sys, B.sys, st);
+        upload(B.mass, mass, st);
+        upload(B.invm, invm, st);
+        upload(B.q, q, st);
+        upload(B.sig, sig, st);
+        upload(B.eps, eps, st);
+        upload(B.atom_sys, atom_sys, st);
+        upload(B.slot_sys, slot_sys, st);
+        upload(B.col_sys, col_sys, st);
+        upload(B.super_sys, super_sys, st);
+        upload(B.excl_off, eoff, st);
+        upload(B.excl_idx, eidx, st);
+        upload(B.excl_pairs, epairs, st);
+        upload(B.bond_idx, bidx, st);
+        upload(B.bond_par, bpar, st);
+        upload(B.angle_idx, aidx, st);
+        upload(B.angle_par, apar, st);
+        upload(B.tors_idx, tidx, st);
+        upload(B.tors_par, tpar, st);
+        upload(B.eta, eta, st);
+        upload(B.twiddle, tw, st);
+        upload(B.fftz_items, zi, st);
+        upload(B.ffty_items, yi, st);
+        upload(B.fftx_items, xi, st);
+        HIPCHK(hipMemsetAsync(B.d_cnt, 0, sizeof(SysCount) * B.nsys, st));
+        HIPCHK(hipMemsetAsync(B.FAx, 0, sizeof(facc) * B.n_atoms, st));
+        HIPCHK(hipMemsetAsync(B.FAy, 0, sizeof(facc) * B.n_atoms, st));
+        HIPCHK(hipMemsetAsync(B.FAz, 0, sizeof(facc) * B.n_atoms, st));
+        HIPCHK(hipMemsetAsync(B.atom_of_slot, 0xff, sizeof(int) * B.n_slots_cap, st));
+        HIPCHK(hipMemsetAsync(B.slot_of_atom, 0, sizeof(int) * B.n_atoms, st));
+        HIPCHK(hipMemsetAsync(B.super_item, 0, sizeof(int4) * B.n_super_cap, st));
+        // positions and velocities
+        std::vector<double> x, y, z, vx, vy, vz;
+        for (int s = 0; s < ns; ++s) {
+            HostSystem& h = *hs_[s];
+            for (int i = 0; i < h.n; ++i) {
+                x.push_back(h.pos[3 * i]); y.push_back(h.pos[3 * i + 1]); z.push_back(h.pos[3 * i + 2]);
+                vx.push_back(h.vel[3 * i]); vy.push_back(h.vel[3 * i + 1]); vz.push_back(h.vel[3 * i + 2]);
+            }
+        }
+        upload(B.X, x, st); upload(B.Y, y, st); upload(B.Z, z, st);
+        upload(B.VX, vx, st); upload(B.VY, vy, st); upload(B.VZ, vz, st);
+        request_rebuild();
+        HIPCHK(hipStreamSynchronize(st));
+        // first build; re-lay out with larger list capacities until the lists fit
+        launch_nlist<P>(B);
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<SysCount> c(ns);
+        copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * ns, hipMemcpyDeviceToHost);
+        bool grow = false;
+        for (int s = 0; s < ns; ++s) {
+            if (c[s].overflow & 2) throw HipError(NX_ERR_CAPACITY, "neighbour-list column exceeds 2048 atoms");
+            if (c[s].overflow) { hs_[s]->list_scale *= 2.0; grow = true; }
+            else if (c[s].list_count > 0.8 * B.sys[s].list_cap || c[s].emask_count > 0.8 * B.sys[s].emask_cap) {
+                hs_[s]->list_scale *= 1.6;
+                grow = true;
+            }
+        }
+        counts_ = c;
+        if (grow) {
+            if (++grow_depth_ > 6) throw HipError(NX_ERR_CAPACITY, "pair list does not fit");
+            std::vector<HostSystem*> keep = hs_;
+            setup(keep);
+        }
+        grow_depth_ = 0;
+    }
+
+    // Host<->device copies go through the engine stream (it does not order
+    // with the null stream), then wait.
+    void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, B.stream));
+        HIPCHK(hipStreamSynchronize(B.stream));
+    }
+
+    void request_rebuild() {
+        std::vector<SysCount> c(B.nsys);
+        copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * B.nsys, hipMemcpyDeviceToHost);
+        for (auto& v : c) { v.rebuild = 1; v.overflow = 0; }
+        copy_sync(B.d_cnt, c.data(), sizeof(SysCount) * B.nsys, hipMemcpyHostToDevice);
+    }
+
+    // ---------------------------------------------------------------- steps
+
+    struct Ev { const char* name; hipEvent_t a, b; };
+    hipEvent_t ev() {
+        if (ev_used_ < evpool_.size()) return evpool_[ev_used_++];
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        evpool_.push_back(e);
+        ev_used_++;
+        return e;
+    }
+    template <class F>
+    void timed(const char* name, F&& f) {
+        if (!B.timing || capturing_) { f(); return; }
+        hipEvent_t a = ev(), b = ev();
+        HIPCHK(hipEventRecord(a, B.stream));
+        f();
+        HIPCHK(hipEventRecord(b, B.stream));
+        evs_.push_back({name, a, b});
+    }
+
+    void forces(int mask, bool energy) {
+        timed("nlist", [&] { launch_nlist<P>(B); });
+        timed("pack", [&] { launch_pack<P>(B); });
+        if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
+        if (mask & NX_FORCE_NONBONDED) {
+            timed("exclusion", [&] { launch_exclusion<P>(B, energy); });
+            timed("pair", [&] { launch_pair<P>(B, energy); });
+            timed("pme_spread", [&] { launch_pme_spread<P>(B); });
+            timed("fft", [&] {
+                launch_fft_forward<P>(B);
+                launch_fft_convolve<P>(B, energy);
+                launch_fft_inverse<P>(B);
+            });
+            timed("pme_gather", [&] { launch_pme_gather<P>(B); });
+        }
+    }
+
+    void zero_energy() { HIPCHK(hipMemsetAsync(B.energy, 0, sizeof(long long) * B.nsys * E_NTERMS, B.stream)); }
+
+    void begin_call() {
+        evs_.clear();
+        ev_used_ = 0;
+        steps_ = 0;
+        HIPCHK(hipGetLastError());
+    }
+
+    void end_call() {
+        HIPCHK(hipStreamSynchronize(B.stream));
+        HIPCHK(hipGetLastError());
+        times_.clear();
+        for (auto& e : evs_) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+            auto& t = times_[e.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        check_counters();
+    }
+
+    void check_counters() {
+        std::vector<SysCount> c(B.nsys);
+        copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * B.nsys, hipMemcpyDeviceToHost);
+        for (int s = 0; s < B.nsys; ++s)
+            if (c[s].overflow) {
+                failed_sys_ = s;
+                throw HipError(NX_ERR_CAPACITY, "neighbour-list capacity exceeded in system " + std::to_string(s) +
+                                                    " (code " + std::to_string(c[s].overflow) + ")");
+            }
+        counts_ = c;
+    }
+
+    void step_plain(double dt, int mask) {
+        forces(mask, false);
+        timed("integrate", [&] { launch_integrate<P>(B, dt); });
+        ++steps_;
+    }
+
+    void run(int nsteps, double dt, int mask) override {
+        begin_call();
+        const bool use_graph = !B.timing && nsteps >= 4 && graphs_enabled_;
+        if (use_graph) {
+            if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
+            for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
+            steps_ += nsteps;
+        } else {
+            for (int i = 0; i < nsteps; ++i) step_plain(dt, mask);
+        }
+        end_call();
+    }
+
+    void build_graph(double dt, int mask) {
+        if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+        if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+        capturing_ = true;
+        HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        forces(mask, false);
+        launch_integrate<P>(B, dt);
+        hipError_t e = hipStreamEndCapture(B.stream, &graph_);
+        capturing_ = false;
+        HIPCHK(e);
+        HIPCHK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        graph_dt_ = dt;
+        graph_mask_ = mask;
+    }
+
+    void energies_out(int s, int mask, nx_energies* e) {
+        std::vector<long long> h(E_NTERMS);
+        copy_sync(h.data(), B.energy + (size_t)s * E_NTERMS, sizeof(long long) * E_NTERMS, hipMemcpyDeviceToHost);
+        auto v = [&](int t) { return (double)h[t] / kEnergyScale; };
+        std::memset(e, 0, sizeof(*e));
+        e->bond = (mask & 1) ? v(E_BOND) : 0.0;
+        e->angle = (mask & 2) ? v(E_ANGLE) : 0.0;
+        e->torsion = (mask & 4) ? v(E_TORSION) : 0.0;
+        e->kinetic = v(E_KIN);
+        const double bonded = e->bond + e->angle + e->torsion;
+        e->potential = bonded;
+        e->reference_pe = bonded;
+        if (mask & NX_FORCE_NONBONDED) {
+            e->coulomb_direct = v(E_COUL);
+            e->lj = v(E_LJ);
+            e->recip = v(E_RECIP);
+            e->self = self_energy_[s];
+            e->exclusion = v(E_EXCL);
+            e->reference_pe = e->recip + e->self + (e->coulomb_direct + e->lj) - e->exclusion;
+            e->potential = bonded + e->recip + e->self + e->coulomb_direct + e->lj - e->exclusion;
+        }
+    }
+
+    // Forces + energies at the current positions; optionally kinetic energy and integrate.
+    void compute(int mask, bool kinetic, double dt, bool integrate, std::vector<double>* F,
+                 std::vector<double>* pos, std::vector<double>* vel, std::vector<nx_energies>* E) override {
+        begin_call();
+        zero_energy();
+        forces(mask, true);
+        if (kinetic) launch_kinetic<P>(B, dt);
+        HIPCHK(hipStreamSynchronize(B.stream));
+        if (F) gather_forces(*F);
+        if (pos || vel) get_state(pos, vel);
+        if (E) {
+            E->resize(B.nsys);
+            for (int s = 0; s < B.nsys; ++s) energies_out(s, mask, &(*E)[s]);
+        }
+        if (integrate) {
+            timed("integrate", [&] { launch_integrate<P>(B, dt); });
+            ++steps_;
+        } else {
+            HIPCHK(hipMemsetAsync(B.FAx, 0, sizeof(facc) * B.n_atoms, B.stream));
+            HIPCHK(hipMemsetAsync(B.FAy, 0, sizeof(facc) * B.n_atoms, B.stream));
+            HIPCHK(hipMemsetAsync(B.FAz, 0, sizeof(facc) * B.n_atoms, B.stream));
+        }
+        end_call();
+    }
+
+    void gather_forces(std::vector<double>& F) {
+        double* d;
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(double) * 3 * (size_t)B.n_atoms, B.stream));
+        launch_collect_forces<P>(B, d, d + B.n_atoms, d + 2 * (size_t)B.n_atoms);
+        std::vector<double> h(3 * (size_t)B.n_atoms);
+        HIPCHK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, B.stream));
+        HIPCHK(hipFreeAsync(d, B.stream));
+        HIPCHK(hipStreamSynchronize(B.stream));
+        F.resize(3 * (size_t)B.n_atoms);
+        for (int a = 0; a < B.n_atoms; ++a)
+            for (int k = 0; k < 3; ++k) F[3 * (size_t)a + k] = h[(size_t)k * B.n_atoms + a];
+    }
+
+    void get_state(std::vector<double>* pos, std::vector<double>* vel) override {
+        const size_t n = B.n_atoms;
+        std::vector<double> h(3 * n);
+        if (pos) {
+            HIPCHK(hipMemcpyAsync(h.data(), B.X, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipMemcpyAsync(h.data() + n, B.Y, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipMemcpyAsync(h.data() + 2 * n, B.Z, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipStreamSynchronize(B.stream));
+            pos->resize(3 * n);
+            for (size_t a = 0; a < n; ++a)
+                for (int k = 0; k < 3; ++k) (*pos)[3 * a + k] = h[k * n + a];
+        }
+        if (vel) {
+            HIPCHK(hipMemcpyAsync(h.data(), B.VX, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipMemcpyAsync(h.data() + n, B.VY, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipMemcpyAsync(h.data() + 2 * n, B.VZ, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipStreamSynchronize(B.stream));
+            vel->resize(3 * n);
+            for (size_t a = 0; a < n; ++a)
+                for (int k = 0; k < 3; ++k) (*vel)[3 * a + k] = h[k * n + a];
+        }
+    }
+
+    void set_state(const double* pos, const double* vel) override {
+        const size_t n = B.n_atoms;
+        std::vector<double> h(3 * n);
+        for (int k = 0; k < 3; ++k)
+            for (size_t a = 0; a < n; ++a) h[k * n + a] = pos[3 * a + k];
+        copy_sync(B.X, h.data(), n * 8, hipMemcpyHostToDevice);
+        copy_sync(B.Y, h.data() + n, n * 8, hipMemcpyHostToDevice);
+        copy_sync(B.Z, h.data() + 2 * n, n * 8, hipMemcpyHostToDevice);
+        for (int k = 0; k < 3; ++k)
+            for (size_t a = 0; a < n; ++a) h[k * n + a] = vel ? vel[3 * a + k] : 0.0;
+        copy_sync(B.VX, h.data(), n * 8, hipMemcpyHostToDevice);
+        copy_sync(B.VY, h.data() + n, n * 8, hipMemcpyHostToDevice);
+        copy_sync(B.VZ, h.data() + 2 * n, n * 8, hipMemcpyHostToDevice);
+        request_rebuild();
+    }
+
+    long long export_pairs(std::vector<int>& out) override {
+        begin_call();
+        // make the list current for these positions
+        launch_nlist<P>(B);
+        HIPCHK(hipStreamSynchronize(B.stream));
+        check_counters();
+        long long cap = 1 << 20;
+        while (true) {
+            int2* d;
+            HIPCHK(hipMalloc(&d, sizeof(int2) * cap));
+            long long n = launch_export_pairs<P>(B, d, cap, d_count_);
+            if (n <= cap) {
+                std::vector<int2> h(n);
+                if (n) copy_sync(h.data(), d, sizeof(int2) * n, hipMemcpyDeviceToHost);
+                hipFree(d);
+                out.resize(2 * n);
+                for (long long i = 0; i < n; ++i) { out[2 * i] = h[i].x; out[2 * i + 1] = h[i].y; }
+                return n;
+            }
+            hipFree(d);
+            cap = n;
+        }
+    }
+
+    void stats(nx_stats* st) override {
+        std::memset(st, 0, sizeof(*st));
+        std::vector<SysCount> c(B.nsys);
+        copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * B.nsys, hipMemcpyDeviceToHost);
+        long long tiles = 0;
+        std::vector<int4> items(B.n_super_cap);
+        copy_sync(items.data(), B.super_item, sizeof(int4) * B.n_super_cap, hipMemcpyDeviceToHost);
+        std::vector<long long> off(B.nsys + 1, 0);
+        for (int s = 0; s < B.nsys; ++s) {
+            st->n_slots += c[s].n_slots;
+            st->n_clusters += c[s].n_clusters;
+            st->n_super += c[s].n_super;
+            st->list_entries += c[s].list_count;
+            st->rebuilds += c[s].rebuilds;
+            for (int i = 0; i < c[s].n_super; ++i) (void)i;
+        }
+        // tiles = sum over entries of popcount(imask): read the list regions
+        for (int s = 0; s < B.nsys; ++s) {
+            const SysDev& d = B.sys[s];
+            std::vector<uint2> l(c[s].list_count);
+            if (!l.empty())
+                copy_sync(l.data(), B.list + d.list_off, sizeof(uint2) * l.size(), hipMemcpyDeviceToHost);
+            for (auto& v : l) tiles += __builtin_popcount(v.x >> 24);
+        }
+        st->tiles = tiles;
+        st->steps = steps_;
+        for (int k = 0; k < 3; ++k) st->grid[k] = B.sys.empty() ? 0 : B.sys[0].grid[k];
+        st->alpha = B.sys.empty() ? 0 : B.sys[0].alpha;
+    }
+
+    bool kernel_time(const std::string& name, double* ms, long long* n) override {
+        auto it = times_.find(name);
+        if (it == times_.end()) return false;
+        *ms = it->second.second ? it->second.first / it->second.second : 0.0;
+        *n = it->second.second;
+        return true;
+    }
+
+    void set_timing(bool on) override { B.timing = on; }
+    void set_graphs(bool on) override { graphs_enabled_ = on; }
+    int nsys() const override { return B.nsys; }
+    int failed_system() const override { return failed_sys_; }
+    const std::vector<SysCount>& counters() const override { return counts_; }
+    hipStream_t stream() const override { return B.stream; }
+    void device_state(int s, double* pos, double* vel) override {
+        const SysDev& d = B.sys[s];
+        const size_t n = d.n_atoms, o = d.atom_off;
+        std::vector<double> h(n);
+        double* dst[6] = {pos, pos, pos, vel, vel, vel};
+        double* src[6] = {B.X, B.Y, B.Z, B.VX, B.VY, B.VZ};
+        for (int k = 0; k < 6; ++k) {
+            if (!dst[k]) continue;
+            HIPCHK(hipMemcpyAsync(h.data(), src[k] + o, n * 8, hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipStreamSynchronize(B.stream));
+            for (size_t a = 0; a < n; ++a) dst[k][3 * a + (k % 3)] = h[a];
+        }
+    }
+    size_t bytes() const override {
+        size_t tot = 0;
+        (void)tot;
+        return footprint_bytes(B);
+    }
+    static size_t footprint_bytes(const Batch<P>& b) {
+        size_t A = b.n_atoms, S = b.n_slots_cap;
+        return A * (14 * 8 + 3 * 4 + 3 * sizeof(facc)) + S * (3 * 4 + sizeof(real4) + sizeof(real2) + 3 * sizeof(facc)) +
+               (size_t)b.n_list_cap * 8 + (size_t)b.n_emask_cap * 64 + (size_t)b.rgrid_total * (sizeof(facc) + sizeof(real)) +
+               (size_t)b.cgrid_total * (sizeof(real2) + sizeof(real));
+    }
+
+   private:
+    int device_;
+    Batch<P> B;
+    std::vector<void*> owned_;
+    std::vector<HostSystem*> hs_;
+    std::vector<double> self_energy_;
+    int fft_maxK_ = 0;
+    long long* d_count_ = nullptr;
+    std::vector<hipEvent_t> evpool_;
+    size_t ev_used_ = 0;
+    std::vector<Ev> evs_;
+    std::map<std::string, std::pair<double, long long>> times_;
+    long long steps_ = 0;
+    bool capturing_ = false;
+    bool graphs_enabled_ = true;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    double graph_dt_ = 0.0;
+    int graph_mask_ = -1;
+    int failed_sys_ = -1;
+    int grow_depth_ = 0;
+    std::vector<SysCount> counts_;
+};
+
+EngineBase* make_engine(int device, int precision) {
+    if (precision == NX_PRECISION_MIXED) return new Engine<PrecMixed>(device);
+    return new Engine<PrecFP64>(device);
+}
+
+}  // namespace nx

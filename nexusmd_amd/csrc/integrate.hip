@@ -1,0 +1,194 @@
+// integrate.hip -- per-step packing, leapfrog update, kinetic energy.
+//
+// Leapfrog as VerletIntegration::Advance (core/baseline/src/
+// VerletIntegration.cpp:49-89): v += F/m dt; x += v dt; one wrap into
+// [lb, ub]; atoms with 1/m = 0 are left untouched.  Kinetic energy with the
+// half-step shift of KineticEnergy::calculateKineticEnergy
+// (core/baseline/src/KineticEnergy.cpp:21-41).  Positions and velocities stay
+// FP64 in both precision paths; the mixed path converts its fixed-point
+// force sums to FP64 here.
+#include "nx_device.h"
+#include "nx_engine.h"
+
+namespace nx {
+
+// Slot-order copy of (x, y, z, q) for the pair and PME kernels; clears F_slot.
+template <class P>
+__global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
+                            const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
+                            const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
+                            const double* __restrict__ Q, typename P::real4* pos, typename P::facc* FX,
+                            typename P::facc* FY, typename P::facc* FZ, int n_slots_total) {
+    using real = typename P::real;
+    const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= n_slots_total) return;
+    const int s = slot_sys[sl];
+    const int a = (sl - sd[s].slot_off < sc[s].n_slots) ? atom_of_slot[sl] : -1;
+    typename P::real4 p;
+    if (a >= 0) {
+        p.x = (real)X[a];
+        p.y = (real)Y[a];
+        p.z = (real)Z[a];
+        p.w = (real)Q[a];
+    } else {
+        const real nan = __builtin_nan("");
+        p.x = p.y = p.z = nan;
+        p.w = 0;
+    }
+    pos[sl] = p;
+    FX[sl] = 0;
+    FY[sl] = 0;
+    FZ[sl] = 0;
+}
+
+template <class P>
+__device__ __forceinline__ void total_force(int a, int sl, const typename P::facc* FAx, const typename P::facc* FAy,
+                                            const typename P::facc* FAz, const typename P::facc* FSx,
+                                            const typename P::facc* FSy, const typename P::facc* FSz, double* f) {
+    if constexpr (P::mixed) {
+        // exact integer sum, then one conversion
+        long long x = FAx[a] + FSx[sl], y = FAy[a] + FSy[sl], z = FAz[a] + FSz[sl];
+        f[0] = (double)x * (1.0 / kForceScale);
+        f[1] = (double)y * (1.0 / kForceScale);
+        f[2] = (double)z * (1.0 / kForceScale);
+    } else {
+        f[0] = FAx[a] + FSx[sl];
+        f[1] = FAy[a] + FSy[sl];
+        f[2] = FAz[a] + FSz[sl];
+    }
+}
+
+template <class P>
+__global__ void kinetic_kernel(const int* __restrict__ atom_sys, const int* __restrict__ slot_of_atom,
+                               const double* __restrict__ VX, const double* __restrict__ VY,
+                               const double* __restrict__ VZ, const double* __restrict__ mass,
+                               const typename P::facc* FAx, const typename P::facc* FAy, const typename P::facc* FAz,
+                               const typename P::facc* FSx, const typename P::facc* FSy, const typename P::facc* FSz,
+                               double dt, long long* energy, int n_atoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    int s = -1;
+    if (a < n_atoms) {
+        s = atom_sys[a];
+        double f[3];
+        total_force<P>(a, slot_of_atom[a], FAx, FAy, FAz, FSx, FSy, FSz, f);
+        double v[3] = {VX[a], VY[a], VZ[a]};
+        const double m = mass[a];
+        if (m > 0) {
+            const double im = 1.0 / m, ts = 0.5 * dt;
+            v[0] += f[0] * im * ts;
+            v[1] += f[1] * im * ts;
+            v[2] += f[2] * im * ts;
+        }
+        ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    // wave-uniform system -> one atomic per wave
+    const int s0 = __shfl(s, 0, 64);
+    const bool uniform = __all(s == s0 || s < 0);
+    if (uniform) {
+        ke = wave_sum(ke);
+        if ((threadIdx.x & 63) == 0 && s0 >= 0) energy_add(&energy[s0 * E_NTERMS + E_KIN], ke);
+    } else if (s >= 0) {
+        energy_add(&energy[s * E_NTERMS + E_KIN], ke);
+    }
+}
+
+template <class P>
+__global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, const int* __restrict__ atom_sys,
+                                 const int* __restrict__ slot_of_atom, double* X, double* Y, double* Z, double* VX,
+                                 double* VY, double* VZ, const double* __restrict__ invm,
+                                 const double* __restrict__ XB, const double* __restrict__ YB,
+                                 const double* __restrict__ ZB, typename P::facc* FAx, typename P::facc* FAy,
+                                 typename P::facc* FAz, const typename P::facc* FSx, const typename P::facc* FSy,
+                                 const typename P::facc* FSz, double dt, int n_atoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    int s = 0;
+    if (a < n_atoms) {
+        s = atom_sys[a];
+        const SysDev& d = sd[s];
+        double f[3];
+        total_force<P>(a, slot_of_atom[a], FAx, FAy, FAz, FSx, FSy, FSz, f);
+        const double im = invm[a];
+        double x = X[a], y = Y[a], z = Z[a];
+        if (im != 0.0) {
+            double vx = VX[a] + im * f[0] * dt;
+            double vy = VY[a] + im * f[1] * dt;
+            double vz = VZ[a] + im * f[2] * dt;
+            double p[3] = {x + vx * dt, y + vy * dt, z + vz * dt};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (p[k] < d.lb[k]) p[k] += d.L[k];
+                else if (p[k] > d.ub[k]) p[k] -= d.L[k];
+            }
+            x = p[0]; y = p[1]; z = p[2];
+            X[a] = x; Y[a] = y; Z[a] = z;
+            VX[a] = vx; VY[a] = vy; VZ[a] = vz;
+        }
+        const double dx = min_image1(x - XB[a], d.L[0]);
+        const double dy = min_image1(y - YB[a], d.L[1]);
+        const double dz = min_image1(z - ZB[a], d.L[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        moved = !(d2 <= d.skin_half2);
+        FAx[a] = 0;
+        FAy[a] = 0;
+        FAz[a] = 0;
+    }
+    // lanes of a wave storing to one flag merge into one write
+    if (moved) sc[s].rebuild = 1;
+}
+
+template <class P>
+__global__ void collect_kernel(const int* __restrict__ slot_of_atom, const typename P::facc* FAx,
+                               const typename P::facc* FAy, const typename P::facc* FAz, const typename P::facc* FSx,
+                               const typename P::facc* FSy, const typename P::facc* FSz, double* FX, double* FY,
+                               double* FZ, int n_atoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_atoms) return;
+    double f[3];
+    total_force<P>(a, slot_of_atom[a], FAx, FAy, FAz, FSx, FSy, FSz, f);
+    FX[a] = f[0];
+    FY[a] = f[1];
+    FZ[a] = f[2];
+}
+
+template <class P>
+void launch_pack(Batch<P>& B) {
+    const int T = 256;
+    pack_kernel<P><<<(unsigned)((B.n_slots_cap + T - 1) / T), T, 0, B.stream>>>(
+        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.X, B.Y, B.Z, B.q, B.pos, B.FSx, B.FSy, B.FSz, B.n_slots_cap);
+}
+
+template <class P>
+void launch_kinetic(Batch<P>& B, double dt) {
+    const int T = 256;
+    kinetic_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
+        B.atom_sys, B.slot_of_atom, B.VX, B.VY, B.VZ, B.mass, B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, dt, B.energy,
+        B.n_atoms);
+}
+
+template <class P>
+void launch_integrate(Batch<P>& B, double dt) {
+    const int T = 256;
+    integrate_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
+        B.d_sys, B.d_cnt, B.atom_sys, B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.invm, B.XB, B.YB, B.ZB, B.FAx,
+        B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, dt, B.n_atoms);
+}
+
+template <class P>
+void launch_collect_forces(Batch<P>& B, double* FX, double* FY, double* FZ) {
+    const int T = 256;
+    collect_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
+        B.slot_of_atom, B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, FX, FY, FZ, B.n_atoms);
+}
+
+template void launch_pack<PrecFP64>(Batch<PrecFP64>&);
+template void launch_pack<PrecMixed>(Batch<PrecMixed>&);
+template void launch_kinetic<PrecFP64>(Batch<PrecFP64>&, double);
+template void launch_kinetic<PrecMixed>(Batch<PrecMixed>&, double);
+template void launch_integrate<PrecFP64>(Batch<PrecFP64>&, double);
+template void launch_integrate<PrecMixed>(Batch<PrecMixed>&, double);
+template void launch_collect_forces<PrecFP64>(Batch<PrecFP64>&, double*, double*, double*);
+template void launch_collect_forces<PrecMixed>(Batch<PrecMixed>&, double*, double*, double*);
+
+}  // namespace nx

@@ -1,0 +1,349 @@
+// nonbonded.hip -- real-space LJ + erfc-Coulomb pair kernel, exclusion
+// correction, and the explicit pair-set export.
+//
+// Physics follows NonbondedForce::DirectForcesAndEnergy and
+// CalculateExclusionEnergyAndForces (core/baseline/src/Forces/
+// NonbondedForce.cpp:530-585, :590-630) with the modified parameters of
+// NonbondedParamsModifier (:504-513): sigma_ij = sig_i/2 + sig_j/2 and
+// eps'_i = 2 sqrt(eps_i), so E_LJ = eps'_i eps'_j (s^6 - 1) s^6.
+//
+// Work decomposition: one wavefront per super-cluster (up to 8 clusters of 8
+// atoms).  Lane l = 8*ia + ja holds i-atom ia of the current i-cluster and
+// j-atom ja of the current j-cluster, so one (i-cluster, j-cluster) tile is
+// 64 pairs in one VALU pass.  i-forces stay in registers over the whole
+// list; j-forces are reduced across the 8 lanes sharing ja and added once
+// per tile column.  The FP64 path tests r^2 < rc^2 with the reference's
+// minimum image and an uncontracted left-to-right dot product, so its pair
+// set is bit-identical to the reference's.
+#include "nx_device.h"
+#include "nx_engine.h"
+
+namespace nx {
+
+// Abramowitz & Stegun 7.1.26: erfc(x) ~ t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2),
+// t = 1 / (1 + p x), |error| <= 1.5e-7 for x >= 0.
+__device__ __forceinline__ float erfc_as(float x, float expmx2) {
+    const float p = 0.3275911f;
+    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
+                a5 = 1.061405429f;
+    float t = __builtin_amdgcn_rcpf(1.0f + p * x);
+    return t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5)))) * expmx2;
+}
+
+struct PairOut {
+    float dEdR_f;
+};
+
+// Mixed path: FP32 pair math.  Returns dE/dr / r (force = d * value on i).
+__device__ __forceinline__ float pair_mixed(float r2, float qq, float sig, float epsij, float alpha, float sw_on,
+                                            float sw_inv, float& ecoul, float& elj) {
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * inv_r;
+    const float inv_r2 = inv_r * inv_r;
+    const float ar = alpha * r;
+    const float ex = __expf(-ar * ar);
+    const float erfcv = erfc_as(ar, ex);
+    const float ce = qq * inv_r * erfcv;
+    const float dc = (ce + qq * (float)kTwoOverSqrtPi * alpha * ex) * inv_r2;
+    const float s2 = sig * sig * inv_r2;
+    const float s6 = s2 * s2 * s2;
+    float le = epsij * (s6 - 1.0f) * s6;
+    float dl = epsij * (12.0f * s6 - 6.0f) * s6 * inv_r2;
+    if (sw_on >= 0.f && r > sw_on) {
+        const float t = (r - sw_on) * sw_inv;
+        const float sv = 1.0f + t * t * t * (-10.0f + t * (15.0f - t * 6.0f));
+        const float sd = t * t * (-30.0f + t * (60.0f - t * 30.0f)) * sw_inv;
+        dl = sv * dl - le * sd * inv_r;
+        le *= sv;
+    }
+    ecoul = ce;
+    elj = le;
+    return dc + dl;
+}
+
+// FP64 path: the reference's formulas with library erfc/exp.
+__device__ __forceinline__ double pair_fp64(double r2, double qq, double sig, double epsij, double alpha,
+                                            double sw_on, double sw_inv, double& ecoul, double& elj) {
+    const double r = sqrt(r2);
+    const double inv_r = 1.0 / r;
+    const double ar = alpha * r;
+    const double erfcv = erfc(ar);
+    const double ex = exp(-ar * ar);
+    double dEdR = qq * inv_r * inv_r * inv_r * (erfcv + 2.0 * ar * ex / kSqrtPi);
+    double s2 = sig * inv_r;
+    s2 *= s2;
+    const double s6 = s2 * s2 * s2;
+    double le = epsij * (s6 - 1.0) * s6;
+    double sv = 1.0, sd = 0.0;
+    if (sw_on >= 0.0 && r > sw_on) {
+        const double t = (r - sw_on) * sw_inv;
+        sv = 1.0 + t * t * t * (-10.0 + t * (15.0 - t * 6.0));
+        sd = t * t * (-30.0 + t * (60.0 - t * 30.0)) * sw_inv;
+    }
+    dEdR += sv * epsij * (12.0 * s6 - 6.0) * s6 * inv_r * inv_r;
+    if (sw_on >= 0.0) {
+        dEdR -= le * sd * inv_r;
+        le *= sv;
+    }
+    ecoul = qq * inv_r * erfcv;
+    elj = le;
+    return dEdR;
+}
+
+template <class P, bool ENERGY>
+__global__ __launch_bounds__(64) void pair_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
+                                                  const int* __restrict__ super_sys,
+                                                  const int4* __restrict__ super_item, const uint2* __restrict__ list,
+                                                  const unsigned long long* __restrict__ emask,
+                                                  const typename P::real4* __restrict__ pos,
+                                                  const typename P::real2* __restrict__ lj, typename P::facc* FX,
+                                                  typename P::facc* FY, typename P::facc* FZ, long long* energy,
+                                                  int n_items) {
+    using real = typename P::real;
+    using real4 = typename P::real4;
+    using real2 = typename P::real2;
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= n_items) return;
+    const int s = super_sys[item];
+    if (item - sd[s].super_off >= sc[s].n_super) return;
+    const int4 it = super_item[item];
+    const int fc = it.x, ncl = it.y, lstart = it.z, llen = it.w;
+    const int lane = threadIdx.x, ia = lane >> 3, ja = lane & 7;
+    const real Lx = (real)sd[s].L[0], Ly = (real)sd[s].L[1], Lz = (real)sd[s].L[2];
+    const real rc2 = (real)sd[s].rc2;
+    const real alpha = (real)sd[s].alpha;
+    const real sw_on = (real)sd[s].sw_on, sw_inv = (real)sd[s].sw_inv;
+    const real ke = (real)kOne4PiEps0;
+
+    real xi[8], yi[8], zi[8], qi[8], si[8], ei[8];
+    real fxi[8], fyi[8], fzi[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k < ncl) {
+            const int slot = (fc + k) * 8 + ia;
+            const real4 p = pos[slot];
+            const real2 l = lj[slot];
+            xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; qi[k] = p.w * ke; si[k] = l.x; ei[k] = l.y;
+        } else {
+            xi[k] = yi[k] = zi[k] = qi[k] = si[k] = ei[k] = 0;
+        }
+        fxi[k] = fyi[k] = fzi[k] = 0;
+    }
+    double ec_acc = 0.0, el_acc = 0.0;
+
+    for (int e = 0; e < llen; ++e) {
+        const uint2 ent = list[lstart + e];
+        const int cj = (int)(ent.x & 0xffffffu);
+        const unsigned im = ent.x >> 24;
+        const unsigned eid = ent.y;
+        const int sj = cj * 8 + ja;
+        const real4 pj = pos[sj];
+        const real2 lj_j = lj[sj];
+        real fxj = 0, fyj = 0, fzj = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (im & (1u << k)) {
+                bool excl = (cj == fc + k) && (ja <= ia);
+                if (eid != 0xffffffffu) excl |= ((emask[(size_t)eid * 8 + k] >> lane) & 1ull) != 0ull;
+                const real dx = min_image1(xi[k] - pj.x, Lx);
+                const real dy = min_image1(yi[k] - pj.y, Ly);
+                const real dz = min_image1(zi[k] - pj.z, Lz);
+                real r2;
+                if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
+                else r2 = dot_rn(dx, dy, dz);
+                if (!excl && r2 < rc2) {
+                    real ecv, elv, dEdR;
+                    if constexpr (P::mixed)
+                        dEdR = pair_mixed(r2, qi[k] * pj.w, si[k] + lj_j.x, ei[k] * lj_j.y, alpha, sw_on, sw_inv,
+                                          ecv, elv);
+                    else
+                        dEdR = pair_fp64(r2, qi[k] * pj.w, si[k] + lj_j.x, ei[k] * lj_j.y, alpha, sw_on, sw_inv,
+                                         ecv, elv);
+                    const real fx = dx * dEdR, fy = dy * dEdR, fz = dz * dEdR;
+                    fxi[k] += fx; fyi[k] += fy; fzi[k] += fz;
+                    fxj -= fx; fyj -= fy; fzj -= fz;
+                    if constexpr (ENERGY) { ec_acc += (double)ecv; el_acc += (double)elv; }
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 8; o < 64; o <<= 1) {
+            fxj += __shfl_xor(fxj, o, 64);
+            fyj += __shfl_xor(fyj, o, 64);
+            fzj += __shfl_xor(fzj, o, 64);
+        }
+        if (ia == 0) {
+            facc_add(&FX[sj], fxj);
+            facc_add(&FY[sj], fyj);
+            facc_add(&FZ[sj], fzj);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k < ncl) {
+            real fx = fxi[k], fy = fyi[k], fz = fzi[k];
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1) {
+                fx += __shfl_xor(fx, o, 64);
+                fy += __shfl_xor(fy, o, 64);
+                fz += __shfl_xor(fz, o, 64);
+            }
+            if (ja == 0) {
+                const int slot = (fc + k) * 8 + ia;
+                facc_add(&FX[slot], fx);
+                facc_add(&FY[slot], fy);
+                facc_add(&FZ[slot], fz);
+            }
+        }
+    }
+    if constexpr (ENERGY) {
+        ec_acc = wave_sum(ec_acc);
+        el_acc = wave_sum(el_acc);
+        if (lane == 0) {
+            energy_add(&energy[s * E_NTERMS + E_COUL], ec_acc);
+            energy_add(&energy[s * E_NTERMS + E_LJ], el_acc);
+        }
+    }
+}
+
+// Exclusion correction over excluded pairs (i < j), forces in atom order.
+template <class P, bool ENERGY>
+__global__ void exclusion_kernel(const SysDev* __restrict__ sd, const int* __restrict__ atom_sys,
+                                 const int2* __restrict__ pairs, int n_pairs, const double* __restrict__ X,
+                                 const double* __restrict__ Y, const double* __restrict__ Z,
+                                 const double* __restrict__ Q, typename P::facc* FX, typename P::facc* FY,
+                                 typename P::facc* FZ, long long* energy) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double eacc = 0.0;
+    int s = -1;
+    if (t < n_pairs) {
+        const int2 pr = pairs[t];
+        const int i = pr.x, j = pr.y;
+        s = atom_sys[i];
+        const SysDev& d = sd[s];
+        const double dxd = min_image1(X[i] - X[j], d.L[0]);
+        const double dyd = min_image1(Y[i] - Y[j], d.L[1]);
+        const double dzd = min_image1(Z[i] - Z[j], d.L[2]);
+        const double qq = kOne4PiEps0 * Q[i] * Q[j];
+        if constexpr (P::mixed) {
+            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float ir = 1.0f / r;
+            const float al = (float)d.alpha;
+            const float ar = al * r;
+            const float ef = erff(ar);
+            const float qf = (float)qq;
+            if (ef > 1e-6f) {
+                const float dEdR = qf * ir * ir * ir * (ef - 2.0f * ar * __expf(-ar * ar) * (float)(1.0 / kSqrtPi));
+                facc_add(&FX[i], -dx * dEdR); facc_add(&FY[i], -dy * dEdR); facc_add(&FZ[i], -dz * dEdR);
+                facc_add(&FX[j], dx * dEdR); facc_add(&FY[j], dy * dEdR); facc_add(&FZ[j], dz * dEdR);
+                eacc = (double)(qf * ir * ef);
+            } else {
+                eacc = d.alpha * kTwoOverSqrtPi * qq;
+            }
+        } else {
+            const double r = sqrt(dxd * dxd + dyd * dyd + dzd * dzd);
+            const double ir = 1.0 / r;
+            const double ar = d.alpha * r;
+            const double ef = erf(ar);
+            if (ef > 1e-6) {
+                const double dEdR = qq * ir * ir * ir * (ef - 2.0 * ar * exp(-ar * ar) / kSqrtPi);
+                facc_add(&FX[i], -dxd * dEdR); facc_add(&FY[i], -dyd * dEdR); facc_add(&FZ[i], -dzd * dEdR);
+                facc_add(&FX[j], dxd * dEdR); facc_add(&FY[j], dyd * dEdR); facc_add(&FZ[j], dzd * dEdR);
+                eacc = qq * ir * ef;
+            } else {
+                eacc = d.alpha * kTwoOverSqrtPi * qq;
+            }
+        }
+    }
+    if constexpr (ENERGY) {
+        // pairs of one block may belong to several systems: add per thread when mixed systems
+        if (s >= 0) energy_add(&energy[s * E_NTERMS + E_EXCL], eacc);
+    }
+}
+
+// Explicit pair set (atom ids, first > second) with the FP64 test of the pair kernel.
+template <class P>
+__global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, const SysCount* sc, const int* super_sys,
+                                                          const int4* super_item, const uint2* list,
+                                                          const unsigned long long* emask, const int* atom_of_slot,
+                                                          const double* X, const double* Y, const double* Z,
+                                                          int2* out, long long cap, unsigned long long* count,
+                                                          int n_items) {
+    const int item = blockIdx.x;
+    if (item >= n_items) return;
+    const int s = super_sys[item];
+    if (item - sd[s].super_off >= sc[s].n_super) return;
+    const int4 it = super_item[item];
+    const int fc = it.x, ncl = it.y, lstart = it.z, llen = it.w;
+    const int lane = threadIdx.x, ia = lane >> 3, ja = lane & 7;
+    const double Lx = sd[s].L[0], Ly = sd[s].L[1], Lz = sd[s].L[2], rc2 = sd[s].rc2;
+    for (int e = 0; e < llen; ++e) {
+        const uint2 ent = list[lstart + e];
+        const int cj = (int)(ent.x & 0xffffffu);
+        const unsigned im = ent.x >> 24;
+        const int aj = atom_of_slot[cj * 8 + ja];
+        for (int k = 0; k < ncl; ++k) {
+            if (!(im & (1u << k))) continue;
+            const int ai = atom_of_slot[(fc + k) * 8 + ia];
+            bool excl = (cj == fc + k) && (ja <= ia);
+            if (ent.y != 0xffffffffu) excl |= ((emask[(size_t)ent.y * 8 + k] >> lane) & 1ull) != 0ull;
+            if (excl || ai < 0 || aj < 0) continue;
+            const double dx = min_image1(X[ai] - X[aj], Lx);
+            const double dy = min_image1(Y[ai] - Y[aj], Ly);
+            const double dz = min_image1(Z[ai] - Z[aj], Lz);
+            if (dot_rn(dx, dy, dz) < rc2) {
+                unsigned long long w = atomicAdd(count, 1ull);
+                if ((long long)w < cap) out[w] = make_int2(max(ai, aj), min(ai, aj));
+            }
+        }
+    }
+}
+
+template <class P>
+void launch_pair(Batch<P>& B, bool energy) {
+    if (B.n_super_cap == 0) return;
+    if (energy)
+        pair_kernel<P, true><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.list,
+                                                                 B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz,
+                                                                 B.energy, B.n_super_cap);
+    else
+        pair_kernel<P, false><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item,
+                                                                  B.list, B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz,
+                                                                  B.energy, B.n_super_cap);
+}
+
+template <class P>
+void launch_exclusion(Batch<P>& B, bool energy) {
+    if (B.n_excl_pairs == 0) return;
+    const int T = 256;
+    const unsigned nb = (unsigned)((B.n_excl_pairs + T - 1) / T);
+    if (energy)
+        exclusion_kernel<P, true><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_pairs, B.n_excl_pairs, B.X, B.Y,
+                                                         B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy);
+    else
+        exclusion_kernel<P, false><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_pairs, B.n_excl_pairs, B.X,
+                                                          B.Y, B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy);
+}
+
+template <class P>
+long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count) {
+    hipMemsetAsync(d_count, 0, sizeof(long long), B.stream);
+    export_pairs_kernel<P><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.list,
+                                                               B.emask, B.atom_of_slot, B.X, B.Y, B.Z, out, cap,
+                                                               reinterpret_cast<unsigned long long*>(d_count),
+                                                               B.n_super_cap);
+    long long n = 0;
+    hipMemcpyAsync(&n, d_count, sizeof(long long), hipMemcpyDeviceToHost, B.stream);
+    hipStreamSynchronize(B.stream);
+    return n;
+}
+
+template void launch_pair<PrecFP64>(Batch<PrecFP64>&, bool);
+template void launch_pair<PrecMixed>(Batch<PrecMixed>&, bool);
+template void launch_exclusion<PrecFP64>(Batch<PrecFP64>&, bool);
+template void launch_exclusion<PrecMixed>(Batch<PrecMixed>&, bool);
+template long long launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);
+template long long launch_export_pairs<PrecMixed>(Batch<PrecMixed>&, int2*, long long, long long*);
+
+}  // namespace nx

@@ -1,0 +1,166 @@
+// nx_device.h -- device-side types and helpers shared by the HIP kernels.
+//
+// Data layout (all systems of a batch packed back to back):
+//   atom order  : FP64 master state x/y/z, vx/vy/vz, 1/m (SoA), per-atom
+//                 force accumulators F_atom (bonded terms, exclusion
+//                 correction), atom_sys, slot_of_atom.
+//   slot order  : atoms binned into xy columns, sorted by z inside a column
+//                 and padded to 8-atom clusters; 8 consecutive clusters of a
+//                 column form a super-cluster (one wavefront of the pair
+//                 kernel).  Per slot: packed (x,y,z,q), (sig/2, 2*sqrt(eps)),
+//                 atom_of_slot (-1 = padding), force accumulators F_slot
+//                 (pair, PME).
+//   PME         : per system a real grid [Kx][Ky][Kz] (fixed-point int64 in
+//                 the mixed path, FP64 otherwise), a half-spectrum complex
+//                 grid [Kx][Ky][Kz/2+1] and a real potential grid.
+// Forces: FP64 path adds doubles with hardware atomics; mixed path converts
+// each contribution to 64-bit fixed point (2^32 units per kJ/mol/nm), so sums
+// are exact integers and independent of order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nx {
+
+constexpr double kOne4PiEps0 = 138.93545764446428693393;   // kJ nm / (mol e^2)
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrtPi = 1.77245385090551602729;
+constexpr double kTwoOverSqrtPi = 1.1283791670955126;
+constexpr double kForceScale = 4294967296.0;               // 2^32 fixed-point units
+constexpr double kEnergyScale = 1073741824.0;              // 2^30 fixed-point units
+constexpr int kClusterSize = 8;
+constexpr int kSuperClusters = 8;                           // clusters per super-cluster
+constexpr int kPmeOrder = 5;
+
+enum EnergyTerm { E_BOND = 0, E_ANGLE, E_TORSION, E_COUL, E_LJ, E_RECIP, E_EXCL, E_KIN, E_NTERMS };
+
+// Per-system constants (device copy; written once per topology/box).
+struct SysDev {
+    double L[3], invL[3], lb[3], ub[3];
+    double rc, rc2, rlist, alpha, skin_half2;
+    double sw_on, sw_inv;        // switching: r_switch, 1/(rc - r_switch); sw_on < 0 = off
+    int n_atoms, atom_off;
+    int slot_off, slot_cap;
+    int col_off, ncx, ncy, col_cap;   // columns: ncx*ncy
+    int super_off, super_cap;
+    int list_off, list_cap;
+    int emask_off, emask_cap;
+    double colw[2];              // column widths
+    int grid[3], pad0;
+    long long rgrid_off, cgrid_off;   // offsets into the real / complex grids
+    int excl_off, n_excl;        // excluded pair list
+    int bond_off, n_bonds, angle_off, n_angles, tors_off, n_tors;
+    int tw_off[3];               // twiddle-table offsets per axis
+};
+
+// Per-system counters written by the device.
+struct SysCount {
+    int n_slots, n_clusters, n_super, list_count;
+    int emask_count, rebuild, overflow, rebuilds;
+};
+
+// ------------------------------------------------------------ precision
+
+struct PrecFP64 {
+    using real = double;
+    using real2 = double2;
+    using real4 = double4;
+    using facc = double;          // force accumulator element
+    static constexpr bool mixed = false;
+};
+
+struct PrecMixed {
+    using real = float;
+    using real2 = float2;
+    using real4 = float4;
+    using facc = long long;       // 2^32 fixed point
+    static constexpr bool mixed = true;
+};
+
+__device__ __forceinline__ long long to_fixed(float v) {
+    float s = v * 4294967296.0f;
+    s = fminf(fmaxf(s, -9.2e18f), 9.2e18f);
+    return __float2ll_rn(s);
+}
+__device__ __forceinline__ long long to_fixed(double v) {
+    double s = v * kForceScale;
+    s = fmin(fmax(s, -9.2e18), 9.2e18);
+    return __double2ll_rn(s);
+}
+
+__device__ __forceinline__ void facc_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void facc_add(long long* p, float v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(to_fixed(v)));
+}
+__device__ __forceinline__ void facc_add(long long* p, double v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(to_fixed(v)));
+}
+__device__ __forceinline__ double facc_get(const double* p) { return *p; }
+__device__ __forceinline__ double facc_get(const long long* p) { return static_cast<double>(*p) * (1.0 / kForceScale); }
+
+// Energy accumulation: block partial (double) -> 2^30 fixed point (deterministic sum).
+__device__ __forceinline__ void energy_add(long long* e, double v) {
+    double s = v * kEnergyScale;
+    s = fmin(fmax(s, -9.2e18), 9.2e18);
+    atomicAdd(reinterpret_cast<unsigned long long*>(e), static_cast<unsigned long long>(__double2ll_rn(s)));
+}
+
+// ---------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide (<=1024 threads) sum of a double into energy slot e (one atomic per block).
+__device__ __forceinline__ void block_energy_add(long long* e, double v) {
+    __shared__ double part[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) t += part[i];
+        energy_add(e, t);
+    }
+    __syncthreads();
+}
+
+// Reference minimum image: one +-L correction (PeriodicBoundaryCondition.cpp:93-104).
+__device__ __forceinline__ double min_image1(double d, double L) {
+    if (d > 0.5 * L) d -= L;
+    else if (d < -0.5 * L) d += L;
+    return d;
+}
+__device__ __forceinline__ float min_image1(float d, float L) {
+    if (d > 0.5f * L) d -= L;
+    else if (d < -0.5f * L) d += L;
+    return d;
+}
+
+// r^2 without contraction, left-to-right as Coords3D::dot (Coords3D.h:116-124).
+__device__ __forceinline__ double dot_rn(double x, double y, double z) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+// XCD-aware block order: consecutive work items land on one XCD (8 XCDs,
+// round-robin dispatch), so neighbouring super-clusters share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+    const int per = (nblocks + 7) >> 3;
+    const int xcd = b & 7, k = b >> 3;
+    const int full = nblocks & 7 ? (nblocks & 7) : 8;   // XCDs that get `per` blocks
+    int base = (xcd < full) ? xcd * per : full * per + (xcd - full) * (per - 1);
+    int r = base + k;
+    return r < nblocks ? r : b;
+}
+
+}  // namespace nx

@@ -1,0 +1,111 @@
+// nx_engine.h -- host-side batch of systems resident on one GPU.
+//
+// A Batch holds every device buffer for one or more independent systems
+// packed back to back (a single-system context is a batch of one).  Kernels
+// take the packed arrays plus per-item system tables, so one launch advances
+// every system of the batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "nx_device.h"
+
+namespace nx {
+
+struct HostSystem;   // engine.hip
+
+template <class P>
+struct Batch {
+    using real = typename P::real;
+    using real2 = typename P::real2;
+    using real4 = typename P::real4;
+    using facc = typename P::facc;
+
+    int nsys = 0;
+    std::vector<SysDev> sys;             // host copy of the descriptors
+    SysDev* d_sys = nullptr;
+    SysCount* d_cnt = nullptr;
+    hipStream_t stream = nullptr;
+
+    int n_atoms = 0, n_slots_cap = 0, n_cols = 0, n_super_cap = 0;
+    long long n_list_cap = 0, n_emask_cap = 0;
+
+    // atom order (FP64 master state)
+    double *X = nullptr, *Y = nullptr, *Z = nullptr;
+    double *VX = nullptr, *VY = nullptr, *VZ = nullptr;
+    double *invm = nullptr, *mass = nullptr, *q = nullptr, *sig = nullptr, *eps = nullptr;
+    double *XB = nullptr, *YB = nullptr, *ZB = nullptr;
+    int *atom_sys = nullptr, *slot_of_atom = nullptr, *atom_col = nullptr;
+    facc *FAx = nullptr, *FAy = nullptr, *FAz = nullptr;
+
+    // slot order
+    int *slot_sys = nullptr, *atom_of_slot = nullptr, *slot_tmp = nullptr;
+    real4* pos = nullptr;
+    real2* lj = nullptr;
+    facc *FSx = nullptr, *FSy = nullptr, *FSz = nullptr;
+
+    // columns / clusters / pair list
+    int *col_sys = nullptr, *col_count = nullptr, *col_fill = nullptr, *col_slot = nullptr, *col_super = nullptr;
+    float4 *cl_ctr = nullptr, *cl_half = nullptr;
+    int* super_sys = nullptr;
+    int4* super_item = nullptr;
+    uint2* list = nullptr;
+    unsigned long long* emask = nullptr;
+    int *excl_off = nullptr, *excl_idx = nullptr;
+    int2* excl_pairs = nullptr;
+    int n_excl_pairs = 0;
+
+    // bonded terms (global atom indices)
+    int2* bond_idx = nullptr;
+    double2* bond_par = nullptr;
+    int n_bonds = 0;
+    int4* angle_idx = nullptr;
+    double2* angle_par = nullptr;
+    int n_angles = 0;
+    int4* tors_idx = nullptr;
+    double4* tors_par = nullptr;   // k, phase, periodicity, unused
+    int n_tors = 0;
+
+    // PME
+    facc* rgrid = nullptr;          // spread charges [sum Kx*Ky*Kz]
+    real2* cgrid = nullptr;         // half spectrum [sum Kx*Ky*(Kz/2+1)]
+    real* phi = nullptr;            // potential grid [sum Kx*Ky*Kz]
+    real* eta = nullptr;            // influence function on the half spectrum
+    real2* twiddle = nullptr;       // per system, per axis exp(-2 pi i t / K)
+    long long rgrid_total = 0, cgrid_total = 0, tw_total = 0;
+    int4* fftz_items = nullptr;     // (sys, first line, n lines, 0)
+    int4* ffty_items = nullptr;     // (sys, x, kz0, 0)
+    int4* fftx_items = nullptr;     // (sys, y, kz0, 0)
+    int n_fftz = 0, n_ffty = 0, n_fftx = 0;
+    int fft_lines = 1, fft_tile = 1, fft_maxK = 0;
+    int pme_any = 0;
+
+    // energies [nsys][E_NTERMS] in 2^30 fixed point
+    long long* energy = nullptr;
+
+    // flags
+    bool timing = false;
+};
+
+template <class P> void launch_nlist(Batch<P>& B);
+template <class P> void launch_pack(Batch<P>& B);
+template <class P> void launch_pair(Batch<P>& B, bool energy);
+template <class P> void launch_exclusion(Batch<P>& B, bool energy);
+template <class P> void launch_bonded(Batch<P>& B, int mask, bool energy);
+template <class P> void launch_pme_spread(Batch<P>& B);
+template <class P> void launch_fft_forward(Batch<P>& B);
+template <class P> void launch_fft_convolve(Batch<P>& B, bool energy);
+template <class P> void launch_fft_inverse(Batch<P>& B);
+template <class P> void launch_pme_gather(Batch<P>& B);
+template <class P> void launch_kinetic(Batch<P>& B, double dt);
+template <class P> void launch_integrate(Batch<P>& B, double dt);
+template <class P> void launch_collect_forces(Batch<P>& B, double* FX, double* FY, double* FZ);
+template <class P> long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count);
+// standalone FFT (verification)
+template <class P> void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool forward,
+                                        hipStream_t st);
+
+}  // namespace nx

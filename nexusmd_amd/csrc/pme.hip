@@ -1,0 +1,591 @@
+// pme.hip -- smooth PME reciprocal space on the device.
+//
+// Follows NonbondedForce::ReciprocalPMEcalculateForcesAndEnergy
+// (core/baseline/src/Forces/NonbondedForce.cpp:467-499): fractional index
+// and offset from the origin (:51-71), order-5 B-splines by the standard
+// recursion (:150-181), charge spreading (:220-260), forward 3D FFT, the
+// influence-function convolution and energy (:323-388), backward FFT and the
+// force gather (:407-461).
+//
+// The reference runs a full complex-to-complex FFTW transform of a real grid.
+// Here the spread grid is real, so the forward transform is real-to-complex
+// onto the half spectrum [Kx][Ky][Kz/2+1] and the backward one complex-to-
+// real; by Hermitian symmetry this is the same result (the reference keeps
+// only the real part after its backward transform).  Five passes, each one
+// read and one write of the grid:
+//   Z  : R2C along z (lines contiguous), fixed-point -> real on load;
+//   Y  : C2C along y on tiles of consecutive kz (coalesced rows);
+//   X  : C2C along x, multiply by the influence function eta(k) and sum the
+//        energy, inverse C2C along x -- fused so the spectrum makes one trip;
+//   Y' : inverse C2C along y;
+//   Z' : C2R along z into the real potential grid.
+// Each line FFT is a Stockham autosort radix-{4,2,3,5} transform in LDS with
+// twiddles from a per-axis table exp(-2 pi i t / K) computed on the host in FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "nx_device.h"
+#include "nx_engine.h"
+
+namespace nx {
+
+template <typename real>
+__device__ __forceinline__ void bspline5(real dr, real* th, real* dth) {
+    // NonbondedForce.cpp:150-181 with pmeOrder = 5
+    th[4] = 0;
+    th[1] = dr;
+    th[0] = 1 - dr;
+#pragma unroll
+    for (int k = 3; k < 5; ++k) {
+        const real div = (real)1.0 / (real)(k - 1.0);
+        th[k - 1] = div * dr * th[k - 2];
+#pragma unroll
+        for (int l = 1; l < (k - 1); ++l)
+            th[k - l - 1] = div * ((dr + l) * th[k - l - 2] + (k - l - dr) * th[k - l - 1]);
+        th[0] = div * (1 - dr) * th[0];
+    }
+    dth[0] = -th[0];
+#pragma unroll
+    for (int k = 1; k < 5; ++k) dth[k] = th[k - 1] - th[k];
+    const real div = (real)1.0 / (real)4.0;
+    th[4] = div * dr * th[3];
+#pragma unroll
+    for (int l = 1; l < 4; ++l) th[4 - l] = div * ((dr + l) * th[4 - l - 1] + (5 - l - dr) * th[4 - l]);
+    th[0] = div * (1 - dr) * th[0];
+}
+
+template <typename real>
+__device__ __forceinline__ void grid_index(real x, double invL, int K, int& idx, real& fr) {
+    real t = x * (real)invL;
+    t = (t - floor(t)) * (real)K;
+    if (!(t >= (real)0)) t = 0;
+    int ti = (int)t;
+    fr = t - (real)ti;
+    idx = ti % K;
+}
+
+__device__ __forceinline__ void grid_add(double* g, double v) { unsafeAtomicAdd(g, v); }
+__device__ __forceinline__ void grid_add(long long* g, float v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(to_fixed(v)));
+}
+
+// One thread per (slot, ix): 25 grid updates each.
+template <class P>
+__global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
+                                  const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
+                                  const typename P::real4* __restrict__ pos, typename P::facc* rgrid,
+                                  int n_slots_total) {
+    using real = typename P::real;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = (int)(t / 5), ix = (int)(t % 5);
+    if (slot >= n_slots_total) return;
+    const int s = slot_sys[slot];
+    const SysDev& d = sd[s];
+    if (slot - d.slot_off >= sc[s].n_slots) return;
+    if (atom_of_slot[slot] < 0) return;
+    if (d.grid[0] == 0) return;
+    const typename P::real4 p = pos[slot];
+    const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
+    int i0, i1, i2;
+    real f0, f1, f2;
+    grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
+    grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
+    grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
+    real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
+    bspline5<real>(f0, tx, dtx);
+    bspline5<real>(f1, ty, dty);
+    bspline5<real>(f2, tz, dtz);
+    const real qx = p.w * tx[ix];
+    const int xi = (i0 + ix) % Kx;
+    typename P::facc* g = rgrid + d.rgrid_off + (long long)xi * Ky * Kz;
+#pragma unroll
+    for (int iy = 0; iy < 5; ++iy) {
+        const int yi = (i1 + iy) % Ky;
+        const real qxy = qx * ty[iy];
+#pragma unroll
+        for (int iz = 0; iz < 5; ++iz) {
+            const int zi = (i2 + iz) % Kz;
+            grid_add(&g[(long long)yi * Kz + zi], qxy * tz[iz]);
+        }
+    }
+}
+
+// One thread per slot: force from the potential grid (NonbondedForce.cpp:407-461).
+template <class P>
+__global__ void pme_gather_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
+                                  const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
+                                  const typename P::real4* __restrict__ pos, const typename P::real* __restrict__ phi,
+                                  typename P::facc* FX, typename P::facc* FY, typename P::facc* FZ,
+                                  int n_slots_total) {
+    using real = typename P::real;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n_slots_total) return;
+    const int s = slot_sys[slot];
+    const SysDev& d = sd[s];
+    if (slot - d.slot_off >= sc[s].n_slots) return;
+    if (atom_of_slot[slot] < 0) return;
+    if (d.grid[0] == 0) return;
+    const typename P::real4 p = pos[slot];
+    const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
+    int i0, i1, i2;
+    real f0, f1, f2;
+    grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
+    grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
+    grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
+    real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
+    bspline5<real>(f0, tx, dtx);
+    bspline5<real>(f1, ty, dty);
+    bspline5<real>(f2, tz, dtz);
+    const real* g = phi + d.rgrid_off;
+    real fx = 0, fy = 0, fz = 0;
+#pragma unroll
+    for (int ix = 0; ix < 5; ++ix) {
+        const int xi = (i0 + ix) % Kx;
+#pragma unroll
+        for (int iy = 0; iy < 5; ++iy) {
+            const int yi = (i1 + iy) % Ky;
+            const real* row = g + ((long long)xi * Ky + yi) * Kz;
+            const real a = dtx[ix] * ty[iy], b = tx[ix] * dty[iy], c = tx[ix] * ty[iy];
+#pragma unroll
+            for (int iz = 0; iz < 5; ++iz) {
+                const real v = row[(i2 + iz) % Kz];
+                fx += a * tz[iz] * v;
+                fy += b * tz[iz] * v;
+                fz += c * dtz[iz] * v;
+            }
+        }
+    }
+    const real q = p.w;
+    facc_add(&FX[slot], -q * (fx * (real)Kx * (real)d.invL[0]));
+    facc_add(&FY[slot], -q * (fy * (real)Ky * (real)d.invL[1]));
+    facc_add(&FZ[slot], -q * (fz * (real)Kz * (real)d.invL[2]));
+}
+
+// ------------------------------------------------------------------ FFT
+
+template <typename R2>
+__device__ __forceinline__ R2 cmul(R2 a, R2 b) {
+    R2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+// Small DFT of radix r in registers; sign -1 forward, +1 backward.
+template <typename real, typename R2>
+__device__ __forceinline__ void dft_small(R2* v, int r, int sign) {
+    if (r == 2) {
+        R2 a = v[0], b = v[1];
+        v[0].x = a.x + b.x; v[0].y = a.y + b.y;
+        v[1].x = a.x - b.x; v[1].y = a.y - b.y;
+    } else if (r == 4) {
+        R2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        R2 s02 = {a0.x + a2.x, a0.y + a2.y}, d02 = {a0.x - a2.x, a0.y - a2.y};
+        R2 s13 = {a1.x + a3.x, a1.y + a3.y}, d13 = {a1.x - a3.x, a1.y - a3.y};
+        // W = sign * i ; W * (x + iy) = sign * (-y + ix)
+        R2 wd = {(real)sign * -d13.y, (real)sign * d13.x};
+        v[0].x = s02.x + s13.x; v[0].y = s02.y + s13.y;
+        v[2].x = s02.x - s13.x; v[2].y = s02.y - s13.y;
+        v[1].x = d02.x + wd.x;  v[1].y = d02.y + wd.y;
+        v[3].x = d02.x - wd.x;  v[3].y = d02.y - wd.y;
+    } else if (r == 3) {
+        const real c1 = (real)-0.5, s1 = (real)sign * (real)0.86602540378443864676;
+        R2 a0 = v[0], a1 = v[1], a2 = v[2];
+        R2 sm = {a1.x + a2.x, a1.y + a2.y}, df = {a1.x - a2.x, a1.y - a2.y};
+        R2 t = {a0.x + c1 * sm.x, a0.y + c1 * sm.y};
+        // i * s1 * df = (-s1 df.y, s1 df.x)
+        v[0].x = a0.x + sm.x; v[0].y = a0.y + sm.y;
+        v[1].x = t.x - s1 * df.y; v[1].y = t.y + s1 * df.x;
+        v[2].x = t.x + s1 * df.y; v[2].y = t.y - s1 * df.x;
+    } else {  // r == 5
+        const real c1 = (real)0.30901699437494742410, c2 = (real)-0.80901699437494742410;
+        const real s1 = (real)sign * (real)0.95105651629515357212, s2 = (real)sign * (real)0.58778525229247312917;
+        R2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4];
+        R2 b1 = {a1.x + a4.x, a1.y + a4.y}, b4 = {a1.x - a4.x, a1.y - a4.y};
+        R2 b2 = {a2.x + a3.x, a2.y + a3.y}, b3 = {a2.x - a3.x, a2.y - a3.y};
+        R2 t1 = {a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y};
+        R2 t2 = {a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y};
+        // u1 = s1 b4 + s2 b3, u2 = s2 b4 - s1 b3 ; X1 = t1 + i u1, X4 = t1 - i u1, X2 = t2 + i u2, X3 = t2 - i u2
+        R2 u1 = {s1 * b4.x + s2 * b3.x, s1 * b4.y + s2 * b3.y};
+        R2 u2 = {s2 * b4.x - s1 * b3.x, s2 * b4.y - s1 * b3.y};
+        v[0].x = a0.x + b1.x + b2.x; v[0].y = a0.y + b1.y + b2.y;
+        v[1].x = t1.x - u1.y; v[1].y = t1.y + u1.x;
+        v[4].x = t1.x + u1.y; v[4].y = t1.y - u1.x;
+        v[2].x = t2.x - u2.y; v[2].y = t2.y + u2.x;
+        v[3].x = t2.x + u2.y; v[3].y = t2.y - u2.x;
+    }
+}
+
+__device__ __forceinline__ int next_radix(int m) {
+    if (m % 4 == 0) return 4;
+    if (m % 2 == 0) return 2;
+    if (m % 3 == 0) return 3;
+    return 5;
+}
+
+// FFT of `lines` lines of length N stored [line][N] in `a`; `b` is scratch.
+// Returns the buffer holding the result.  All threads of the block call it.
+template <typename real, typename R2>
+__device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* __restrict__ tw, int sign) {
+    int Ns = 1;
+    int rem = N;
+    R2* src = a;
+    R2* dst = b;
+    while (rem > 1) {
+        const int r = next_radix(rem);
+        const int nb = N / r;
+        const int step = N / (Ns * r);
+        for (int w = threadIdx.x; w < lines * nb; w += blockDim.x) {
+            const int line = w / nb, j = w - line * nb;
+            const R2* sp = src + line * N;
+            R2* dp = dst + line * N;
+            const int k = j % Ns;
+            R2 v[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+                if (q < r) v[q] = sp[j + q * nb];
+            if (k != 0) {
+#pragma unroll
+                for (int q = 1; q < 5; ++q) {
+                    if (q < r) {
+                        R2 t = tw[(k * q * step) % N];
+                        if (sign > 0) t.y = -t.y;
+                        v[q] = cmul(v[q], t);
+                    }
+                }
+            }
+            dft_small<real, R2>(v, r, sign);
+            const int base = (j / Ns) * Ns * r + k;
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+                if (q < r) dp[base + q * Ns] = v[q];
+        }
+        __syncthreads();
+        R2* t = src;
+        src = dst;
+        dst = t;
+        Ns *= r;
+        rem /= r;
+    }
+    return src;
+}
+
+__device__ __forceinline__ float fixed_to_real(long long v, float) { return (float)((double)v * (1.0 / kForceScale)); }
+__device__ __forceinline__ double fixed_to_real(double v, double) { return v; }
+
+// Z pass forward: R2C.  item = (sys, first line, n lines)
+template <class P>
+__global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, const int4* __restrict__ items,
+                                                 const typename P::facc* __restrict__ rgrid,
+                                                 typename P::real2* cgrid, const typename P::real2* __restrict__ tw) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const SysDev& d = sd[it.x];
+    const int Kz = d.grid[2], Kzh = Kz / 2 + 1;
+    const int line0 = it.y, nl = it.z;
+    R2* A = smem;
+    R2* B = smem + nl * Kz;
+    const typename P::facc* g = rgrid + d.rgrid_off + (long long)line0 * Kz;
+    for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) {
+        R2 v;
+        v.x = fixed_to_real(g[i], (real)0);
+        v.y = 0;
+        A[i] = v;
+    }
+    __syncthreads();
+    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, tw + d.tw_off[2], -1);
+    R2* c = cgrid + d.cgrid_off + (long long)line0 * Kzh;
+    for (int i = threadIdx.x; i < nl * Kzh; i += blockDim.x) {
+        const int t = i / Kzh, kz = i - t * Kzh;
+        c[i] = res[t * Kz + kz];
+    }
+}
+
+// Z pass inverse: C2R into the potential grid.
+template <class P>
+__global__ __launch_bounds__(256) void fft_z_inv(const SysDev* __restrict__ sd, const int4* __restrict__ items,
+                                                 const typename P::real2* __restrict__ cgrid, typename P::real* phi,
+                                                 const typename P::real2* __restrict__ tw) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const SysDev& d = sd[it.x];
+    const int Kz = d.grid[2], Kzh = Kz / 2 + 1;
+    const int line0 = it.y, nl = it.z;
+    R2* A = smem;
+    R2* B = smem + nl * Kz;
+    const R2* c = cgrid + d.cgrid_off + (long long)line0 * Kzh;
+    for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) {
+        const int t = i / Kz, z = i - t * Kz;
+        R2 v;
+        if (z < Kzh) v = c[t * Kzh + z];
+        else { v = c[t * Kzh + (Kz - z)]; v.y = -v.y; }
+        A[i] = v;
+    }
+    __syncthreads();
+    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, tw + d.tw_off[2], +1);
+    real* o = phi + d.rgrid_off + (long long)line0 * Kz;
+    for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) o[i] = res[i].x;
+}
+
+// Y pass: tile (sys, x, kz0) of TZ consecutive kz columns, lines along y.
+template <class P>
+__global__ __launch_bounds__(256) void fft_y(const SysDev* __restrict__ sd, const int4* __restrict__ items,
+                                             typename P::real2* cgrid, const typename P::real2* __restrict__ tw,
+                                             int TZ, int sign) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const SysDev& d = sd[it.x];
+    const int Ky = d.grid[1], Kzh = d.grid[2] / 2 + 1;
+    const int x = it.y, kz0 = it.z;
+    const int nt = min(TZ, Kzh - kz0);
+    R2* A = smem;
+    R2* B = smem + TZ * Ky;
+    R2* c = cgrid + d.cgrid_off + (long long)x * Ky * Kzh + kz0;
+    for (int i = threadIdx.x; i < nt * Ky; i += blockDim.x) {
+        const int y = i / nt, t = i - y * nt;
+        A[t * Ky + y] = c[(long long)y * Kzh + t];
+    }
+    __syncthreads();
+    R2* res = fft_lines_lds<real, R2>(A, B, Ky, nt, tw + d.tw_off[1], sign);
+    for (int i = threadIdx.x; i < nt * Ky; i += blockDim.x) {
+        const int y = i / nt, t = i - y * nt;
+        c[(long long)y * Kzh + t] = res[t * Ky + y];
+    }
+}
+
+// X pass: forward FFT along x, convolution with eta (and energy), inverse FFT.
+template <class P, bool ENERGY>
+__global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd, const int4* __restrict__ items,
+                                                  typename P::real2* cgrid, const typename P::real* __restrict__ eta,
+                                                  const typename P::real2* __restrict__ tw, int TZ,
+                                                  long long* energy) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const int s = it.x;
+    const SysDev& d = sd[s];
+    const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2], Kzh = Kz / 2 + 1;
+    const int y = it.y, kz0 = it.z;
+    const int nt = min(TZ, Kzh - kz0);
+    R2* A = smem;
+    R2* B = smem + TZ * Kx;
+    const long long sx = (long long)Ky * Kzh;
+    R2* c = cgrid + d.cgrid_off + (long long)y * Kzh + kz0;
+    const real* et = eta + d.cgrid_off + (long long)y * Kzh + kz0;
+    for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
+        const int x = i / nt, t = i - x * nt;
+        A[t * Kx + x] = c[x * sx + t];
+    }
+    __syncthreads();
+    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, tw + d.tw_off[0], -1);
+    double e = 0.0;
+    for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
+        const int x = i / nt, t = i - x * nt;
+        const real w = eta ? et[x * sx + t] : (real)0;
+        R2 v = res[t * Kx + x];
+        if constexpr (ENERGY) {
+            const int kz = kz0 + t;
+            const double wt = (kz == 0 || (2 * kz == Kz)) ? 1.0 : 2.0;
+            e += wt * (double)w * ((double)v.x * v.x + (double)v.y * v.y);
+        }
+        v.x *= w;
+        v.y *= w;
+        res[t * Kx + x] = v;
+    }
+    __syncthreads();
+    R2* other = (res == A) ? B : A;
+    R2* res2 = fft_lines_lds<real, R2>(res, other, Kx, nt, tw + d.tw_off[0], +1);
+    for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
+        const int x = i / nt, t = i - x * nt;
+        c[x * sx + t] = res2[t * Kx + x];
+    }
+    if constexpr (ENERGY) block_energy_add(&energy[s * E_NTERMS + E_RECIP], 0.5 * e);
+}
+
+// Plain X pass (no convolution) for the standalone transform.
+template <class P>
+__global__ __launch_bounds__(256) void fft_x_plain(const SysDev* __restrict__ sd, const int4* __restrict__ items,
+                                                   typename P::real2* cgrid, const typename P::real2* __restrict__ tw,
+                                                   int TZ, int sign) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const SysDev& d = sd[it.x];
+    const int Kx = d.grid[0], Ky = d.grid[1], Kzh = d.grid[2] / 2 + 1;
+    const int y = it.y, kz0 = it.z;
+    const int nt = min(TZ, Kzh - kz0);
+    R2* A = smem;
+    R2* B = smem + TZ * Kx;
+    const long long sx = (long long)Ky * Kzh;
+    R2* c = cgrid + d.cgrid_off + (long long)y * Kzh + kz0;
+    for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
+        const int x = i / nt, t = i - x * nt;
+        A[t * Kx + x] = c[x * sx + t];
+    }
+    __syncthreads();
+    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, tw + d.tw_off[0], sign);
+    for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
+        const int x = i / nt, t = i - x * nt;
+        c[x * sx + t] = res[t * Kx + x];
+    }
+}
+
+template <class T>
+__global__ void fill_zero(T* p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+template <class P>
+void launch_pme_spread(Batch<P>& B) {
+    if (!B.pme_any) return;
+    const int T = 256;
+    fill_zero<typename P::facc><<<(unsigned)((B.rgrid_total + T - 1) / T), T, 0, B.stream>>>(B.rgrid, B.rgrid_total);
+    const long long nthr = (long long)B.n_slots_cap * 5;
+    pme_spread_kernel<P><<<(unsigned)((nthr + T - 1) / T), T, 0, B.stream>>>(B.d_sys, B.d_cnt, B.slot_sys,
+                                                                             B.atom_of_slot, B.pos, B.rgrid,
+                                                                             B.n_slots_cap);
+}
+
+template <class P>
+static size_t fft_smem(const Batch<P>& B, int lines, int N) {
+    return (size_t)2 * lines * N * sizeof(typename P::real2);
+}
+
+template <class P>
+void launch_fft_forward(Batch<P>& B) {
+    if (!B.pme_any) return;
+    fft_z_fwd<P><<<B.n_fftz, 256, fft_smem(B, B.fft_lines, B.fft_maxK), B.stream>>>(B.d_sys, B.fftz_items, B.rgrid,
+                                                                                   B.cgrid, B.twiddle);
+    fft_y<P><<<B.n_ffty, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(B.d_sys, B.ffty_items, B.cgrid,
+                                                                              B.twiddle, B.fft_tile, -1);
+}
+
+template <class P>
+void launch_fft_convolve(Batch<P>& B, bool energy) {
+    if (!B.pme_any) return;
+    if (energy)
+        fft_x_conv<P, true><<<B.n_fftx, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(
+            B.d_sys, B.fftx_items, B.cgrid, B.eta, B.twiddle, B.fft_tile, B.energy);
+    else
+        fft_x_conv<P, false><<<B.n_fftx, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(
+            B.d_sys, B.fftx_items, B.cgrid, B.eta, B.twiddle, B.fft_tile, B.energy);
+}
+
+template <class P>
+void launch_fft_inverse(Batch<P>& B) {
+    if (!B.pme_any) return;
+    fft_y<P><<<B.n_ffty, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(B.d_sys, B.ffty_items, B.cgrid,
+                                                                              B.twiddle, B.fft_tile, +1);
+    fft_z_inv<P><<<B.n_fftz, 256, fft_smem(B, B.fft_lines, B.fft_maxK), B.stream>>>(B.d_sys, B.fftz_items, B.cgrid,
+                                                                                   B.phi, B.twiddle);
+}
+
+template <class P>
+void launch_pme_gather(Batch<P>& B) {
+    if (!B.pme_any) return;
+    const int T = 256;
+    pme_gather_kernel<P><<<(unsigned)((B.n_slots_cap + T - 1) / T), T, 0, B.stream>>>(
+        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.pos, B.phi, B.FSx, B.FSy, B.FSz, B.n_slots_cap);
+}
+
+template void launch_pme_spread<PrecFP64>(Batch<PrecFP64>&);
+template void launch_pme_spread<PrecMixed>(Batch<PrecMixed>&);
+template void launch_fft_forward<PrecFP64>(Batch<PrecFP64>&);
+template void launch_fft_forward<PrecMixed>(Batch<PrecMixed>&);
+template void launch_fft_convolve<PrecFP64>(Batch<PrecFP64>&, bool);
+template void launch_fft_convolve<PrecMixed>(Batch<PrecMixed>&, bool);
+template void launch_fft_inverse<PrecFP64>(Batch<PrecFP64>&);
+template void launch_fft_inverse<PrecMixed>(Batch<PrecMixed>&);
+template void launch_pme_gather<PrecFP64>(Batch<PrecFP64>&);
+template void launch_pme_gather<PrecMixed>(Batch<PrecMixed>&);
+
+// Standalone 3D transform of one real grid through the same pass kernels.
+template <class P>
+void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool forward, hipStream_t st) {
+    using R2 = typename P::real2;
+    using real = typename P::real;
+    const int nzh = nz / 2 + 1;
+    SysDev d{};
+    d.grid[0] = nx; d.grid[1] = ny; d.grid[2] = nz;
+    d.rgrid_off = 0; d.cgrid_off = 0;
+    d.tw_off[0] = 0; d.tw_off[1] = nx; d.tw_off[2] = nx + ny;
+    std::vector<R2> tw(nx + ny + nz);
+    const int K[3] = {nx, ny, nz};
+    for (int a = 0, o = 0; a < 3; o += K[a], ++a)
+        for (int t = 0; t < K[a]; ++t) {
+            const double ang = -2.0 * kPi * (double)t / (double)K[a];
+            tw[o + t].x = (real)cos(ang);
+            tw[o + t].y = (real)sin(ang);
+        }
+    const int maxK = std::max(nx, std::max(ny, nz));
+    const int T = std::max(1, std::min(16, (int)(61440 / (2 * maxK * (int)sizeof(R2)))));
+    std::vector<int4> zi, yi, xi;
+    for (int l = 0; l < nx * ny; l += T) zi.push_back(make_int4(0, l, std::min(T, nx * ny - l), 0));
+    for (int x = 0; x < nx; ++x)
+        for (int k = 0; k < nzh; k += T) yi.push_back(make_int4(0, x, k, 0));
+    for (int y = 0; y < ny; ++y)
+        for (int k = 0; k < nzh; k += T) xi.push_back(make_int4(0, y, k, 0));
+    SysDev* dd; R2* dtw; int4 *dz, *dy, *dx; typename P::facc* rg; R2* cg; real* ph;
+    const size_t G = (size_t)nx * ny * nz, Gh = (size_t)nx * ny * nzh;
+    hipMalloc(&dd, sizeof(SysDev));
+    hipMalloc(&dtw, tw.size() * sizeof(R2));
+    hipMalloc(&dz, zi.size() * sizeof(int4));
+    hipMalloc(&dy, yi.size() * sizeof(int4));
+    hipMalloc(&dx, xi.size() * sizeof(int4));
+    hipMalloc(&rg, G * sizeof(typename P::facc));
+    hipMalloc(&cg, Gh * sizeof(R2));
+    hipMalloc(&ph, G * sizeof(real));
+    hipMemcpyAsync(dd, &d, sizeof(SysDev), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dtw, tw.data(), tw.size() * sizeof(R2), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dz, zi.data(), zi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dy, yi.data(), yi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dx, xi.data(), xi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+    const size_t sm = (size_t)2 * T * maxK * sizeof(R2);
+    if (forward) {
+        std::vector<typename P::facc> h(G);
+        for (size_t i = 0; i < G; ++i) {
+            if constexpr (P::mixed) h[i] = (long long)llrint(in[i] * kForceScale);
+            else h[i] = in[i];
+        }
+        hipMemcpyAsync(rg, h.data(), G * sizeof(typename P::facc), hipMemcpyHostToDevice, st);
+        fft_z_fwd<P><<<(unsigned)zi.size(), 256, sm, st>>>(dd, dz, rg, cg, dtw);
+        fft_y<P><<<(unsigned)yi.size(), 256, sm, st>>>(dd, dy, cg, dtw, T, -1);
+        fft_x_plain<P><<<(unsigned)xi.size(), 256, sm, st>>>(dd, dx, cg, dtw, T, -1);
+        std::vector<R2> hc(Gh);
+        hipMemcpyAsync(hc.data(), cg, Gh * sizeof(R2), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        for (size_t i = 0; i < Gh; ++i) { out[2 * i] = hc[i].x; out[2 * i + 1] = hc[i].y; }
+    } else {
+        std::vector<R2> hc(Gh);
+        for (size_t i = 0; i < Gh; ++i) { hc[i].x = (real)in[2 * i]; hc[i].y = (real)in[2 * i + 1]; }
+        hipMemcpyAsync(cg, hc.data(), Gh * sizeof(R2), hipMemcpyHostToDevice, st);
+        fft_x_plain<P><<<(unsigned)xi.size(), 256, sm, st>>>(dd, dx, cg, dtw, T, +1);
+        fft_y<P><<<(unsigned)yi.size(), 256, sm, st>>>(dd, dy, cg, dtw, T, +1);
+        fft_z_inv<P><<<(unsigned)zi.size(), 256, sm, st>>>(dd, dz, cg, ph, dtw);
+        std::vector<real> hr(G);
+        hipMemcpyAsync(hr.data(), ph, G * sizeof(real), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        for (size_t i = 0; i < G; ++i) out[i] = hr[i];
+    }
+    hipFree(dd); hipFree(dtw); hipFree(dz); hipFree(dy); hipFree(dx); hipFree(rg); hipFree(cg); hipFree(ph);
+}
+
+template void fft3d_roundtrip<PrecFP64>(int, int, int, const double*, double*, bool, hipStream_t);
+template void fft3d_roundtrip<PrecMixed>(int, int, int, const double*, double*, bool, hipStream_t);
+
+}  // namespace nx
