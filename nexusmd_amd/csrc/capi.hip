@@ -290,7 +290,7 @@ nx_status nx_neighbor_pairs(nx_ctx* c, int* pairs, int64_t cap, int64_t* count) 
         if (cap < 0 || (cap > 0 && !pairs)) throw ArgError("bad pair buffer");
         ensure(c);
         std::vector<int> out;
-        long long n = c->eng->export_pairs(out);
+        long long n = c->eng->export_pairs(cap > 0 ? &out : nullptr);
         if (count) *count = n;
         const long long w = std::min<long long>(n, cap);
         if (w > 0) std::memcpy(pairs, out.data(), sizeof(int) * 2 * (size_t)w);

@@ -522,12 +522,26 @@ class Engine : public EngineBase {
     }
     template <class F>
     void timed(const char* name, F&& f) {
-        if (!B.timing || capturing_) { f(); return; }
+        if (!B.timing) { f(); return; }
+        // inside a capture the records become event nodes of the graph
         hipEvent_t a = ev(), b = ev();
         HIPCHK(hipEventRecord(a, B.stream));
         f();
         HIPCHK(hipEventRecord(b, B.stream));
         evs_.push_back({name, a, b});
+    }
+
+    // Fold the elapsed times of recorded events into times_ and recycle the events.
+    void harvest() {
+        for (auto& e : evs_) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+            auto& t = times_[e.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        evs_.clear();
+        ev_used_ = 0;
     }
 
     void forces(int mask, bool energy) {
@@ -552,6 +566,7 @@ class Engine : public EngineBase {
     void begin_call() {
         evs_.clear();
         ev_used_ = 0;
+        times_.clear();
         steps_ = 0;
         HIPCHK(hipGetLastError());
     }
@@ -559,14 +574,7 @@ class Engine : public EngineBase {
     void end_call() {
         HIPCHK(hipStreamSynchronize(B.stream));
         HIPCHK(hipGetLastError());
-        times_.clear();
-        for (auto& e : evs_) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
-            auto& t = times_[e.name];
-            t.first += ms;
-            t.second += 1;
-        }
+        harvest();
         check_counters();
     }
 
@@ -590,8 +598,11 @@ class Engine : public EngineBase {
 
     void run(int nsteps, double dt, int mask) override {
         begin_call();
-        const bool use_graph = !B.timing && nsteps >= 4 && graphs_enabled_;
-        if (use_graph) {
+        if (B.timing) {
+            // per-kernel HIP events need plain launches (events recorded in a graph
+            // cannot be queried for elapsed time); the host enqueues ahead of the GPU
+            for (int i = 0; i < nsteps; ++i) step_plain(dt, mask);
+        } else if (nsteps >= 4 && graphs_enabled_) {
             if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
             for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
             steps_ += nsteps;
@@ -716,23 +727,24 @@ class Engine : public EngineBase {
         request_rebuild();
     }
 
-    long long export_pairs(std::vector<int>& out) override {
+    long long export_pairs(std::vector<int>* out) override {
         begin_call();
         // make the list current for these positions
         launch_nlist<P>(B);
         HIPCHK(hipStreamSynchronize(B.stream));
         check_counters();
-        long long cap = 1 << 20;
+        long long cap = out ? (1 << 20) : 0;
         while (true) {
-            int2* d;
-            HIPCHK(hipMalloc(&d, sizeof(int2) * cap));
+            int2* d = nullptr;
+            HIPCHK(hipMalloc(&d, sizeof(int2) * std::max<long long>(cap, 1)));
             long long n = launch_export_pairs<P>(B, d, cap, d_count_);
+            if (!out) { hipFree(d); return n; }
             if (n <= cap) {
                 std::vector<int2> h(n);
                 if (n) copy_sync(h.data(), d, sizeof(int2) * n, hipMemcpyDeviceToHost);
                 hipFree(d);
-                out.resize(2 * n);
-                for (long long i = 0; i < n; ++i) { out[2 * i] = h[i].x; out[2 * i + 1] = h[i].y; }
+                out->resize(2 * n);
+                for (long long i = 0; i < n; ++i) { (*out)[2 * i] = h[i].x; (*out)[2 * i + 1] = h[i].y; }
                 return n;
             }
             hipFree(d);

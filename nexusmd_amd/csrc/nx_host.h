@@ -44,7 +44,7 @@ class EngineBase {
                          std::vector<double>* pos, std::vector<double>* vel, std::vector<nx_energies>* E) = 0;
     virtual void get_state(std::vector<double>* pos, std::vector<double>* vel) = 0;
     virtual void set_state(const double* pos, const double* vel) = 0;
-    virtual long long export_pairs(std::vector<int>& out) = 0;
+    virtual long long export_pairs(std::vector<int>* out) = 0;   // nullptr: count only
     virtual void stats(nx_stats* st) = 0;
     virtual bool kernel_time(const std::string& name, double* ms, long long* n) = 0;
     virtual void set_timing(bool on) = 0;

@@ -141,6 +141,12 @@ class Context:
                 return buf[:2 * cnt.value].reshape(-1, 2)
             cap = int(cnt.value)
 
+    def pair_count(self):
+        """Number of within-cutoff non-excluded pairs at the current positions."""
+        cnt = C.c_int64(0)
+        self._check(self.lib.nx_neighbor_pairs(self.h, None, 0, C.byref(cnt)), "nx_neighbor_pairs")
+        return int(cnt.value)
+
     def enable_timing(self, on=True):
         self._check(self.lib.nx_enable_timing(self.h, 1 if on else 0), "nx_enable_timing")
 

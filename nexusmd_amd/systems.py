@@ -13,8 +13,10 @@ particles (q, sig, eps) with cutoff and ewaldTolerance.
 
 Generator choices that the configurations leave open are fixed here:
 bead mass 12.011 amu, minimum non-excluded bead distance 0.3 nm in the
-free chain, water hydrogens carry sig = 0.1 nm and eps = 0 (LJ on oxygen
-only), velocities Maxwell-Boltzmann with the centre-of-mass motion removed.
+free chain, velocities Maxwell-Boltzmann with the centre-of-mass motion
+removed.  In the pure water box the hydrogens carry sig = 0.1 nm, eps = 0
+(LJ on oxygen only); next to charged beads they carry the CHARMM TIP3P core
+sig = 0.04 nm, eps = 0.192 kJ/mol (see water_topology).
 """
 from __future__ import annotations
 
@@ -295,7 +297,14 @@ def _lattice_sites(L, count_min, rng, jitter=0.02, keep=None):
         k += 1
 
 
-def water_topology(n_mol: int, offset: int):
+H_SIG_CORE, H_EPS_CORE = 0.04, 0.192   # CHARMM TIP3P hydrogen
+
+
+def water_topology(n_mol: int, offset: int, h_core: bool = False):
+    """Flexible 3-site water.  h_core gives the hydrogens the small LJ core of the
+    CHARMM TIP3P model; used beside charged beads, whose N(0, 0.4) charges reach
+    |q| ~ 1.5 and would otherwise pull a bare hydrogen onto the bead (with LJ on
+    oxygen only the potential is unbounded below along that path)."""
     o = offset + 3 * np.arange(n_mol, dtype=np.int32)
     b = np.concatenate([np.stack([o, o + 1], 1), np.stack([o, o + 2], 1)]).astype(np.int32)
     order = np.argsort(np.concatenate([o * 2, o * 2 + 1]), kind="stable")
@@ -304,8 +313,9 @@ def water_topology(n_mol: int, offset: int):
     a = np.stack([o + 1, o, o + 2], 1).astype(np.int32)
     ap = np.tile([HOH_A0, HOH_K], (n_mol, 1))
     q = np.tile([Q_O, Q_H, Q_H], n_mol)
-    sig = np.tile([SIG_O, 0.1, 0.1], n_mol)
-    eps = np.tile([EPS_O, 0.0, 0.0], n_mol)
+    hs, he = (H_SIG_CORE, H_EPS_CORE) if h_core else (0.1, 0.0)
+    sig = np.tile([SIG_O, hs, hs], n_mol)
+    eps = np.tile([EPS_O, he, he], n_mol)
     mass = np.tile([O_MASS, H_MASS, H_MASS], n_mol)
     return b, bp, a, ap, q, sig, eps, mass
 
@@ -352,7 +362,7 @@ def solvated_chain(n_chain=10000, n_water=30000, L=10.0, seed=20261204, cutoff=1
     pos = np.concatenate([chain, wat]) % Lv
     b, bp, a, ap, t, tp, tper = chain_topology(n_chain, rng)
     q, sig, eps = chain_nonbonded(n_chain, rng)
-    wb, wbp, wa, wap, wq, wsig, weps, wmass = water_topology(n_water, n_chain)
+    wb, wbp, wa, wap, wq, wsig, weps, wmass = water_topology(n_water, n_chain, h_core=True)
     mass = np.concatenate([np.full(n_chain, BEAD_MASS), wmass])
     vel = maxwell_boltzmann(mass, temperature, rng)
     return System(name or "solvated_chain_%d" % pos.shape[0], pos, vel, mass,
@@ -385,7 +395,7 @@ def ligand_in_water(n_total: int, seed: int, cutoff=0.9, ewald_tol=5e-4, spacing
     pos = np.concatenate([lig, _water_molecules(sites, rng, L=Lv, others=lig)]) % Lv
     b, bp, a, ap, t, tp, tper = chain_topology(n_lig, rng)
     q, sig, eps = chain_nonbonded(n_lig, rng)
-    wb, wbp, wa, wap, wq, wsig, weps, wmass = water_topology(n_wat, n_lig)
+    wb, wbp, wa, wap, wq, wsig, weps, wmass = water_topology(n_wat, n_lig, h_core=True)
     mass = np.concatenate([np.full(n_lig, BEAD_MASS), wmass])
     vel = maxwell_boltzmann(mass, temperature, rng)
     k = int(2 * math.ceil(L / spacing / 2))
