@@ -44,9 +44,12 @@ __global__ void bond_kernel(const SysDev* __restrict__ sd, const int* __restrict
                             typename P::facc* FX, typename P::facc* FY, typename P::facc* FZ, long long* energy) {
     using real = typename P::real;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const bool valid = t < n;
+    int s = 0;
+    double eterm = 0.0;
+    if (valid) {
     const int2 ij = idx[t];
-    const int s = atom_sys[ij.x];
+    s = atom_sys[ij.x];
     const SysDev& d = sd[s];
     V3<real> dl = mi<real>(d, X, Y, Z, ij.y, ij.x);   // pos2 - pos1
     const double2 pk = par[t];
@@ -60,7 +63,9 @@ __global__ void bond_kernel(const SysDev* __restrict__ sd, const int* __restrict
     }
     add3<P>(FX, FY, FZ, ij.x, f1);
     add3<P>(FX, FY, FZ, ij.y, V3<real>{-f1.x, -f1.y, -f1.z});
-    if constexpr (ENERGY) energy_add(&energy[s * E_NTERMS + E_BOND], 0.5 * pk.y * (double)dI * (double)dI);
+    eterm = 0.5 * pk.y * (double)dI * (double)dI;
+    }
+    if constexpr (ENERGY) wave_energy_add(energy, s, E_BOND, eterm, valid);
 }
 
 template <class P, bool ENERGY>
@@ -70,9 +75,12 @@ __global__ void angle_kernel(const SysDev* __restrict__ sd, const int* __restric
                              typename P::facc* FX, typename P::facc* FY, typename P::facc* FZ, long long* energy) {
     using real = typename P::real;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const bool valid = t < n;
+    int s = 0;
+    double eterm = 0.0;
+    if (valid) {
     const int4 ijk = idx[t];
-    const int s = atom_sys[ijk.x];
+    s = atom_sys[ijk.x];
     const SysDev& d = sd[s];
     const V3<real> v0 = mi<real>(d, X, Y, Z, ijk.y, ijk.x);   // p2 - p1
     const V3<real> v1 = mi<real>(d, X, Y, Z, ijk.y, ijk.z);   // p2 - p3
@@ -95,7 +103,9 @@ __global__ void angle_kernel(const SysDev* __restrict__ sd, const int* __restric
     add3<P>(FX, FY, FZ, ijk.x, f1);
     add3<P>(FX, FY, FZ, ijk.y, f2);
     add3<P>(FX, FY, FZ, ijk.z, f3);
-    if constexpr (ENERGY) energy_add(&energy[s * E_NTERMS + E_ANGLE], 0.5 * pk.y * (double)dI * (double)dI);
+    eterm = 0.5 * pk.y * (double)dI * (double)dI;
+    }
+    if constexpr (ENERGY) wave_energy_add(energy, s, E_ANGLE, eterm, valid);
 }
 
 template <typename real>
@@ -113,9 +123,12 @@ __global__ void torsion_kernel(const SysDev* __restrict__ sd, const int* __restr
                                typename P::facc* FZ, long long* energy) {
     using real = typename P::real;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const bool valid = t < n;
+    int s = 0;
+    double eterm = 0.0;
+    if (valid) {
     const int4 q = idx[t];
-    const int s = atom_sys[q.x];
+    s = atom_sys[q.x];
     const SysDev& d = sd[s];
     const V3<real> v0 = mi<real>(d, X, Y, Z, q.x, q.y);   // r1 - r2
     const V3<real> v1 = mi<real>(d, X, Y, Z, q.z, q.y);   // r3 - r2
@@ -150,7 +163,9 @@ __global__ void torsion_kernel(const SysDev* __restrict__ sd, const int* __restr
     add3<P>(FX, FY, FZ, q.y, V3<real>{sv.x - F1.x, sv.y - F1.y, sv.z - F1.z});
     add3<P>(FX, FY, FZ, q.z, V3<real>{-sv.x - F4.x, -sv.y - F4.y, -sv.z - F4.z});
     add3<P>(FX, FY, FZ, q.w, F4);
-    if constexpr (ENERGY) energy_add(&energy[s * E_NTERMS + E_TORSION], pk.x * (1.0 + cos((double)delta)));
+    eterm = pk.x * (1.0 + cos((double)delta));
+    }
+    if constexpr (ENERGY) wave_energy_add(energy, s, E_TORSION, eterm, valid);
 }
 
 template <class P>

@@ -35,6 +35,8 @@ static void dalloc(T** p, size_t n, std::vector<void*>& owned) {
     if (n == 0) n = 1;
     HIPCHK(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
     owned.push_back(*p);
+    // zeroed, so capacity-overflow gaps in work arrays read as empty items
+    HIPCHK(hipMemset(*p, 0, n * sizeof(T)));
 }
 
 template <class T>
@@ -169,7 +171,7 @@ class Engine : public EngineBase {
         const int ns = (int)systems.size();
         B.nsys = ns;
         B.sys.assign(ns, SysDev{});
-        long long atom_off = 0, slot_off = 0, col_off = 0, sup_off = 0, list_off = 0, emask_off = 0;
+        long long atom_off = 0, slot_off = 0, col_off = 0, sup_off = 0, list_off = 0, emask_off = 0, chunk_off = 0;
         long long rg_off = 0, cg_off = 0, tw_off = 0, excl_off_total = 0;
         int bond_off = 0, angle_off = 0, tors_off = 0;
         fft_maxK_ = 0;
@@ -227,6 +229,8 @@ class Engine : public EngineBase {
             per_super = std::max(per_super, 16.0);
             d.list_off = (int)list_off;
             d.list_cap = (int)std::min<double>(2.0e9, per_super * d.super_cap + 64);
+            d.chunk_off = (int)chunk_off;
+            d.chunk_cap = d.list_cap / kChunkEntries + d.super_cap;
             d.emask_off = (int)emask_off;
             d.emask_cap = (int)(std::max<long long>(256, (long long)h.excl_idx.size() / 2 + 64) * h.list_scale);
             // PME grid
@@ -244,6 +248,34 @@ class Engine : public EngineBase {
                 tw_off += g[k];
                 fft_maxK_ = std::max(fft_maxK_, g[k]);
             }
+            // spread tiling: column blocks b x b and nz slabs whose LDS region fits 7680 points
+            {
+                d.sp_margin = 0.5 * skin + 0.002;
+                long long best = -1;
+                int bb = 0, bnz = 0;
+                for (int b = 1; b <= std::min(12, std::max(ncx, ncy)); ++b) {
+                    for (int nz = 1; nz <= g[2]; ++nz) {
+                        auto ext = [&](double w, int K, double Lk, bool full) {
+                            if (full) return K;
+                            const double hh = Lk / K;
+                            int n = (int)ceil((w + 2 * d.sp_margin) / hh) + 6;
+                            return n >= K ? K : n;
+                        };
+                        const int nx = ext(b * d.colw[0], g[0], d.L[0], b >= ncx);
+                        const int ny = ext(b * d.colw[1], g[1], d.L[1], b >= ncy);
+                        const int nzp = ext(d.L[2] / nz, g[2], d.L[2], nz == 1);
+                        const long long pts = (long long)nx * ny * nzp;
+                        if (pts > 7680) continue;
+                        const long long tiles = (long long)((ncx + b - 1) / b) * ((ncy + b - 1) / b) * nz;
+                        const long long cost = pts * tiles + 2000 * (tiles < 128 ? 128 - tiles : 0);
+                        if (best < 0 || cost < best) { best = cost; bb = b; bnz = nz; }
+                        break;   // the smallest fitting nz for this b
+                    }
+                }
+                d.sp_block[0] = bb;
+                d.sp_block[1] = bb;
+                d.sp_block[2] = bnz;
+            }
             rg_off += (long long)g[0] * g[1] * g[2];
             cg_off += (long long)g[0] * g[1] * (g[2] / 2 + 1);
             d.bond_off = bond_off;
@@ -260,6 +292,7 @@ class Engine : public EngineBase {
             col_off += (long long)ncx * ncy;
             sup_off += d.super_cap;
             list_off += d.list_cap;
+            chunk_off += d.chunk_cap;
             emask_off += d.emask_cap;
             excl_off_total += (long long)h.excl_idx.size();
         }
@@ -268,6 +301,7 @@ class Engine : public EngineBase {
         B.n_slots_cap = (int)slot_off;
         B.n_cols = (int)col_off;
         B.n_super_cap = (int)sup_off;
+        B.n_chunk_cap = (int)chunk_off;
         B.n_list_cap = list_off;
         B.n_emask_cap = emask_off;
         B.rgrid_total = rg_off;
@@ -308,6 +342,10 @@ class Engine : public EngineBase {
         dalloc(&B.cl_half, S / 8 + 1, o);
         dalloc(&B.super_sys, B.n_super_cap, o);
         dalloc(&B.super_item, B.n_super_cap, o);
+        dalloc(&B.chunk_sys, B.n_chunk_cap, o);
+        dalloc(&B.slot_zkey, S, o);
+        dalloc(&B.chunks, B.n_chunk_cap, o);
+        dalloc(&B.chunk_prefix, B.nsys + 1, o);
         dalloc(&B.list, (size_t)B.n_list_cap, o);
         dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o);
         dalloc(&B.rgrid, (size_t)B.rgrid_total, o);
@@ -320,20 +358,22 @@ class Engine : public EngineBase {
         if (B.n_bonds) { dalloc(&B.bond_idx, B.n_bonds, o); dalloc(&B.bond_par, B.n_bonds, o); }
         if (B.n_angles) { dalloc(&B.angle_idx, B.n_angles, o); dalloc(&B.angle_par, B.n_angles, o); }
         if (B.n_tors) { dalloc(&B.tors_idx, B.n_tors, o); dalloc(&B.tors_par, B.n_tors, o); }
+        HIPCHK(hipDeviceSynchronize());
     }
 
     void upload_all() {
         hipStream_t st = B.stream;
         const int ns = B.nsys;
         std::vector<double> mass, invm, q, sig, eps;
-        std::vector<int> atom_sys, slot_sys, col_sys, super_sys, eoff(1, 0), eidx;
+        std::vector<int> atom_sys, slot_sys, col_sys, super_sys, chunk_sys, eoff(1, 0), eidx;
         std::vector<int2> epairs, bidx;
         std::vector<double2> bpar, apar;
         std::vector<int4> aidx, tidx;
         std::vector<double4> tpar;
         std::vector<real> eta;
         std::vector<real2> tw;
-        std::vector<int4> zi, yi, xi;
+        std::vector<int4> zi, yi, xi, sp_tiles;
+        int direct_slots = 0;
         const int maxK = fft_maxK_;
         const int T = std::max(1, std::min(16, (int)(61440 / (2 * maxK * (int)sizeof(real2)))));
         B.fft_lines = T;
@@ -362,6 +402,7 @@ class Engine : public EngineBase {
             for (int i = 0; i < d.slot_cap; ++i) slot_sys.push_back(s);
             for (int i = 0; i < d.ncx * d.ncy; ++i) col_sys.push_back(s);
             for (int i = 0; i < d.super_cap; ++i) super_sys.push_back(s);
+            for (int i = 0; i < d.chunk_cap; ++i) chunk_sys.push_back(s);
             for (auto& b : h.bonds) { bidx.push_back(make_int2(a0 + b.p1, a0 + b.p2)); bpar.push_back(make_double2(b.d, b.k)); }
             for (auto& a : h.angles) {
                 aidx.push_back(make_int4(a0 + a.p1, a0 + a.p2, a0 + a.p3, 0));
@@ -408,6 +449,14 @@ class Engine : public EngineBase {
                     v.y = (real)sin(ang);
                     tw.push_back(v);
                 }
+            if (d.sp_block[0] > 0) {
+                const int b = d.sp_block[0], nz = d.sp_block[2];
+                for (int bx = 0; bx < (d.ncx + b - 1) / b; ++bx)
+                    for (int by = 0; by < (d.ncy + b - 1) / b; ++by)
+                        for (int bz = 0; bz < nz; ++bz) sp_tiles.push_back(make_int4(s, bx, by, bz));
+            } else {
+                direct_slots += d.slot_cap;
+            }
             for (int l = 0; l < Kx * Ky; l += T) zi.push_back(make_int4(s, l, std::min(T, Kx * Ky - l), 0));
             for (int x = 0; x < Kx; ++x)
                 for (int k = 0; k < Kzh; k += T) yi.push_back(make_int4(s, x, k, 0));
@@ -422,6 +471,10 @@ class Engine : public EngineBase {
         dalloc(&B.excl_off, eoff.size(), owned_);
         dalloc(&B.excl_idx, std::max<size_t>(1, eidx.size()), owned_);
         if (B.n_excl_pairs) dalloc(&B.excl_pairs, epairs.size(), owned_);
+        B.n_spread_tiles = (int)sp_tiles.size();
+        B.n_spread_direct_slots = direct_slots;
+        dalloc(&B.spread_tiles, std::max<size_t>(1, sp_tiles.size()), owned_);
+        upload(B.spread_tiles, sp_tiles, st);
         dalloc(&B.fftz_items, zi.size(), owned_);
         dalloc(&B.ffty_items, yi.size(), owned_);
         dalloc(&B.fftx_items, xi.size(), owned_);
@@ -438,6 +491,7 @@ class Engine : public EngineBase {
         upload(B.slot_sys, slot_sys, st);
         upload(B.col_sys, col_sys, st);
         upload(B.super_sys, super_sys, st);
+        upload(B.chunk_sys, chunk_sys, st);
         upload(B.excl_off, eoff, st);
         upload(B.excl_idx, eidx, st);
         upload(B.excl_pairs, epairs, st);

@@ -112,6 +112,7 @@ __global__ void nl_scan(const SysDev* sd, SysCount* sc, const int* col_count, in
         sc[s].n_super = ovf ? 0 : tot_sup;
         sc[s].list_count = 0;
         sc[s].emask_count = 0;
+        sc[s].n_chunks = 0;
         if (ovf) sc[s].overflow |= 1;
     }
 }
@@ -133,7 +134,8 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
                                                   const int* col_count, const int* col_slot, const int* col_super,
                                                   const int* slot_atom_tmp, const double* X, const double* Y,
                                                   const double* Z, int* atom_of_slot, int* slot_of_atom,
-                                                  float4* cl_ctr, float4* cl_half, int4* super_item) {
+                                                  float4* cl_ctr, float4* cl_half, int4* super_item,
+                                                  float* slot_zkey) {
     const int col = blockIdx.x;
     const int s = col_sys[col];
     if (!sc[s].rebuild || (sc[s].overflow & 1)) return;
@@ -184,6 +186,10 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
         if (i < cnt) {
             atom_of_slot[slot] = ids[i];
             slot_of_atom[ids[i]] = slot;
+            // round down so that a key never exceeds the true z (slab binning stays monotone)
+            float zk = (float)key[i];
+            if ((double)zk > key[i]) zk = nextafterf(zk, -1.0f);
+            slot_zkey[slot] = zk;
         } else {
             atom_of_slot[slot] = -1;
         }
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(64) void nl_pairlist(const SysDev* sd, SysCount* sc
                                                   const float4* cl_ctr, const float4* cl_half,
                                                   const int* atom_of_slot, const int* slot_of_atom,
                                                   const int* excl_off, const int* excl_idx, uint2* list,
-                                                  unsigned long long* emask, int n_items_total) {
+                                                  unsigned long long* emask, int4* chunks, int n_items_total) {
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= n_items_total) return;
     const int s = super_sys[item];
@@ -404,13 +410,16 @@ __global__ __launch_bounds__(64) void nl_pairlist(const SysDev* sd, SysCount* sc
             atomicOr(&emask[(size_t)g * 8 + k], x_bits[t]);
         }
     }
-    __shared__ int lbase;
+    __shared__ int lbase, cbase;
+    const int nch = (ne + kChunkEntries - 1) / kChunkEntries;
     if (lane == 0) {
         int b = atomicAdd(&sc[s].list_count, ne);
         lbase = (b + ne > d.list_cap) ? -1 : b;
+        int c = atomicAdd(&sc[s].n_chunks, nch);
+        cbase = (c + nch > d.chunk_cap) ? -1 : c;
     }
     __syncthreads();
-    if (lbase < 0) overflow = true;
+    if (lbase < 0 || cbase < 0) overflow = true;
     if (overflow) {
         if (lane == 0) atomicOr(&sc[s].overflow, 4);
         return;
@@ -425,6 +434,11 @@ __global__ __launch_bounds__(64) void nl_pairlist(const SysDev* sd, SysCount* sc
         it.z = d.list_off + lbase;
         it.w = ne;
         super_item[item] = it;
+    }
+    for (int c = lane; c < nch; c += 64) {
+        const int e0 = d.list_off + lbase + c * kChunkEntries;
+        const int e1 = d.list_off + lbase + min(ne, (c + 1) * kChunkEntries);
+        chunks[d.chunk_off + cbase + c] = make_int4(fc, ncl, e0, e1);
     }
 }
 
@@ -482,10 +496,10 @@ void launch_nlist(Batch<P>& B) {
                                               B.slot_tmp, B.n_atoms);
     nl_columns<<<B.n_cols, 256, 0, st>>>(B.d_sys, B.d_cnt, B.col_sys, B.col_count, B.col_slot, B.col_super,
                                          B.slot_tmp, B.X, B.Y, B.Z, B.atom_of_slot, B.slot_of_atom, B.cl_ctr,
-                                         B.cl_half, B.super_item);
+                                         B.cl_half, B.super_item, B.slot_zkey);
     nl_pairlist<<<B.n_super_cap, 64, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
                                               B.cl_ctr, B.cl_half, B.atom_of_slot, B.slot_of_atom, B.excl_off,
-                                              B.excl_idx, B.list, B.emask, B.n_super_cap);
+                                              B.excl_idx, B.list, B.emask, B.chunks, B.n_super_cap);
     nl_finish_slots<P><<<nb(B.n_slots_cap, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.sig,
                                                           B.eps, B.lj, B.n_slots_cap);
     nl_finish_atoms<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.XB, B.YB, B.ZB,

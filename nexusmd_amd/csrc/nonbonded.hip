@@ -15,6 +15,8 @@
 // per tile column.  The FP64 path tests r^2 < rc^2 with the reference's
 // minimum image and an uncontracted left-to-right dot product, so its pair
 // set is bit-identical to the reference's.
+#include <algorithm>
+
 #include "nx_device.h"
 #include "nx_engine.h"
 
@@ -49,8 +51,8 @@ __device__ __forceinline__ float pair_mixed(float r2, float qq, float sig, float
     const float s6 = s2 * s2 * s2;
     float le = epsij * (s6 - 1.0f) * s6;
     float dl = epsij * (12.0f * s6 - 6.0f) * s6 * inv_r2;
-    if (sw_on >= 0.f && r > sw_on) {
-        const float t = (r - sw_on) * sw_inv;
+    if (sw_on >= 0.f) {   // uniform per system; t = 0 inside r_switch gives sv = 1, sd = 0
+        const float t = fmaxf(r - sw_on, 0.0f) * sw_inv;
         const float sv = 1.0f + t * t * t * (-10.0f + t * (15.0f - t * 6.0f));
         const float sd = t * t * (-30.0f + t * (60.0f - t * 30.0f)) * sw_inv;
         dl = sv * dl - le * sd * inv_r;
@@ -90,119 +92,199 @@ __device__ __forceinline__ double pair_fp64(double r2, double qq, double sig, do
     return dEdR;
 }
 
+// Wave-local ordering of LDS traffic (lanes of one wavefront exchanging data
+// through LDS; the other wavefronts of the block run independently).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Chunk prefix over systems: work item g of the whole batch maps to system s
+// with prefix[s] <= g < prefix[s+1] and chunk sd[s].chunk_off + g - prefix[s].
+__global__ void pair_prefix_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc, int nsys,
+                                   int* prefix) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int t = 0;
+    for (int s = 0; s < nsys; ++s) {
+        prefix[s] = t;
+        // the counter runs past the capacity when a build overflows
+        t += min(sc[s].n_chunks, sd[s].chunk_cap);
+    }
+    prefix[nsys] = t;
+}
+
+constexpr int kPairWaves = 4;   // independent wavefronts per 256-thread block
+
 template <class P, bool ENERGY>
-__global__ __launch_bounds__(64) void pair_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
-                                                  const int* __restrict__ super_sys,
-                                                  const int4* __restrict__ super_item, const uint2* __restrict__ list,
-                                                  const unsigned long long* __restrict__ emask,
-                                                  const typename P::real4* __restrict__ pos,
-                                                  const typename P::real2* __restrict__ lj, typename P::facc* FX,
-                                                  typename P::facc* FY, typename P::facc* FZ, long long* energy,
-                                                  int n_items) {
+__global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd, const int* __restrict__ prefix,
+                                                   int nsys, const int4* __restrict__ chunks,
+                                                   const uint2* __restrict__ list,
+                                                   const unsigned long long* __restrict__ emask,
+                                                   const typename P::real4* __restrict__ pos,
+                                                   const typename P::real2* __restrict__ lj, typename P::facc* FX,
+                                                   typename P::facc* FY, typename P::facc* FZ, long long* energy) {
     using real = typename P::real;
     using real4 = typename P::real4;
     using real2 = typename P::real2;
-    const int item = xcd_remap(blockIdx.x, gridDim.x);
-    if (item >= n_items) return;
-    const int s = super_sys[item];
-    if (item - sd[s].super_off >= sc[s].n_super) return;
-    const int4 it = super_item[item];
-    const int fc = it.x, ncl = it.y, lstart = it.z, llen = it.w;
-    const int lane = threadIdx.x, ia = lane >> 3, ja = lane & 7;
-    const real Lx = (real)sd[s].L[0], Ly = (real)sd[s].L[1], Lz = (real)sd[s].L[2];
-    const real rc2 = (real)sd[s].rc2;
-    const real alpha = (real)sd[s].alpha;
-    const real sw_on = (real)sd[s].sw_on, sw_inv = (real)sd[s].sw_inv;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, ia = lane >> 3, ja = lane & 7;
+    __shared__ real4 ipos_s[kPairWaves][64];
+    __shared__ real2 ilj_s[kPairWaves][64];
+    __shared__ real jf_s[kPairWaves][kChunkEntries * 3 * 8];
+    __shared__ int jc_s[kPairWaves][kChunkEntries];
+    real4* ipos = ipos_s[w];
+    real2* ilj = ilj_s[w];
+    real* jf = jf_s[w];
+    int* jc = jc_s[w];
+
+    // contiguous range of work items per wavefront; blocks of one XCD take adjacent ranges
+    const int total = prefix[nsys];
+    const int NW = gridDim.x * kPairWaves;
+    const int vw = xcd_remap(blockIdx.x, gridDim.x) * kPairWaves + w;
+    const int per = (total + NW - 1) / NW;
+    const int g0 = vw * per, g1 = min(total, g0 + per);
     const real ke = (real)kOne4PiEps0;
 
-    real xi[8], yi[8], zi[8], qi[8], si[8], ei[8];
-    real fxi[8], fyi[8], fzi[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        if (k < ncl) {
-            const int slot = (fc + k) * 8 + ia;
-            const real4 p = pos[slot];
-            const real2 l = lj[slot];
-            xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; qi[k] = p.w * ke; si[k] = l.x; ei[k] = l.y;
-        } else {
-            xi[k] = yi[k] = zi[k] = qi[k] = si[k] = ei[k] = 0;
+    for (int g = g0; g < g1; ++g) {
+        int lo = 0, hi = nsys - 1;     // largest s with prefix[s] <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
         }
-        fxi[k] = fyi[k] = fzi[k] = 0;
-    }
-    double ec_acc = 0.0, el_acc = 0.0;
-
-    for (int e = 0; e < llen; ++e) {
-        const uint2 ent = list[lstart + e];
-        const int cj = (int)(ent.x & 0xffffffu);
-        const unsigned im = ent.x >> 24;
-        const unsigned eid = ent.y;
-        const int sj = cj * 8 + ja;
-        const real4 pj = pos[sj];
-        const real2 lj_j = lj[sj];
-        real fxj = 0, fyj = 0, fzj = 0;
+        const int s = lo;
+        const int4 ch = chunks[sd[s].chunk_off + (g - prefix[s])];
+        const int fc = ch.x, ncl = ch.y, e0 = ch.z, e1 = ch.w;
+        const real Lx = (real)sd[s].L[0], Ly = (real)sd[s].L[1], Lz = (real)sd[s].L[2];
+        const real rc2 = (real)sd[s].rc2;
+        const real alpha = (real)sd[s].alpha;
+        const real sw_on = (real)sd[s].sw_on, sw_inv = (real)sd[s].sw_inv;
+        {
+            const int k = lane >> 3;
+            real4 p;
+            real2 l;
+            if (k < ncl) {
+                p = pos[fc * 8 + lane];
+                l = lj[fc * 8 + lane];
+                p.w *= ke;
+            } else {
+                p.x = p.y = p.z = p.w = 0;
+                l.x = l.y = 0;
+            }
+            ipos[lane] = p;
+            ilj[lane] = l;
+        }
+        wave_lds_sync();
+        real fxi[8], fyi[8], fzi[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (im & (1u << k)) {
-                bool excl = (cj == fc + k) && (ja <= ia);
-                if (eid != 0xffffffffu) excl |= ((emask[(size_t)eid * 8 + k] >> lane) & 1ull) != 0ull;
-                const real dx = min_image1(xi[k] - pj.x, Lx);
-                const real dy = min_image1(yi[k] - pj.y, Ly);
-                const real dz = min_image1(zi[k] - pj.z, Lz);
-                real r2;
-                if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
-                else r2 = dot_rn(dx, dy, dz);
-                if (!excl && r2 < rc2) {
+        for (int k = 0; k < 8; ++k) fxi[k] = fyi[k] = fzi[k] = 0;
+        double ec_acc = 0.0, el_acc = 0.0;
+
+        // software pipeline: entry e+1 (list word and j data) loads while entry e computes
+        uint2 ent = list[e0];
+        real4 pj = pos[(int)(ent.x & 0xffffffu) * 8 + ja];
+        real2 lj_j = lj[(int)(ent.x & 0xffffffu) * 8 + ja];
+        for (int e = e0; e < e1; ++e) {
+            const int cj = (int)(ent.x & 0xffffffu);
+            const unsigned im = ent.x >> 24;
+            const unsigned eid = ent.y;
+            uint2 ent_n = ent;
+            real4 pj_n = pj;
+            real2 lj_n = lj_j;
+            if (e + 1 < e1) {
+                ent_n = list[e + 1];
+                const int sjn = (int)(ent_n.x & 0xffffffu) * 8 + ja;
+                pj_n = pos[sjn];
+                lj_n = lj[sjn];
+            }
+            real fxj = 0, fyj = 0, fzj = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (im & (1u << k)) {
+                    bool excl = (cj == fc + k) && (ja <= ia);
+                    if (eid != 0xffffffffu) excl |= ((emask[(size_t)eid * 8 + k] >> lane) & 1ull) != 0ull;
+                    const real4 pi = ipos[k * 8 + ia];
+                    const real2 li = ilj[k * 8 + ia];
+                    const real dx = min_image1(pi.x - pj.x, Lx);
+                    const real dy = min_image1(pi.y - pj.y, Ly);
+                    const real dz = min_image1(pi.z - pj.z, Lz);
+                    real r2;
+                    if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
+                    else r2 = dot_rn(dx, dy, dz);
+                    const bool on = !excl && r2 < rc2;
                     real ecv, elv, dEdR;
-                    if constexpr (P::mixed)
-                        dEdR = pair_mixed(r2, qi[k] * pj.w, si[k] + lj_j.x, ei[k] * lj_j.y, alpha, sw_on, sw_inv,
+                    if constexpr (P::mixed) {
+                        // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
+                        dEdR = pair_mixed(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on, sw_inv,
                                           ecv, elv);
-                    else
-                        dEdR = pair_fp64(r2, qi[k] * pj.w, si[k] + lj_j.x, ei[k] * lj_j.y, alpha, sw_on, sw_inv,
-                                         ecv, elv);
-                    const real fx = dx * dEdR, fy = dy * dEdR, fz = dz * dEdR;
+                        dEdR = on ? dEdR : 0.0f;
+                    } else {
+                        dEdR = 0;
+                        ecv = elv = 0;
+                        if (on)
+                            dEdR = pair_fp64(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
+                                             sw_inv, ecv, elv);
+                    }
+                    // off lanes may hold NaN displacements (padding slots): select both factors
+                    const real fx = (on ? dx : (real)0) * dEdR, fy = (on ? dy : (real)0) * dEdR,
+                               fz = (on ? dz : (real)0) * dEdR;
                     fxi[k] += fx; fyi[k] += fy; fzi[k] += fz;
                     fxj -= fx; fyj -= fy; fzj -= fz;
-                    if constexpr (ENERGY) { ec_acc += (double)ecv; el_acc += (double)elv; }
+                    if constexpr (ENERGY) {
+                        if (on) { ec_acc += (double)ecv; el_acc += (double)elv; }
+                    }
+                }
+            }
+            fxj = sum_stride8(fxj);
+            fyj = sum_stride8(fyj);
+            fzj = sum_stride8(fzj);
+            // staged in LDS: no global atomics inside the loop, so waiting for the
+            // prefetched loads never waits behind atomics (one in-order vmem counter)
+            const int el = e - e0;
+            if (ia == 0) {
+                jf[(el * 3 + 0) * 8 + ja] = fxj;
+                jf[(el * 3 + 1) * 8 + ja] = fyj;
+                jf[(el * 3 + 2) * 8 + ja] = fzj;
+            }
+            if (lane == 0) jc[el] = cj;
+            ent = ent_n;
+            pj = pj_n;
+            lj_j = lj_n;
+        }
+        wave_lds_sync();
+        {
+            // j-forces of the chunk, 8 entries (64 lanes) per atomic instruction
+            const int n = e1 - e0;
+            for (int b = 0; b < n; b += 8) {
+                const int el = b + ia;
+                if (el < n) {
+                    const int sj = jc[el] * 8 + ja;
+                    facc_add(&FX[sj], jf[(el * 3 + 0) * 8 + ja]);
+                    facc_add(&FY[sj], jf[(el * 3 + 1) * 8 + ja]);
+                    facc_add(&FZ[sj], jf[(el * 3 + 2) * 8 + ja]);
                 }
             }
         }
 #pragma unroll
-        for (int o = 8; o < 64; o <<= 1) {
-            fxj += __shfl_xor(fxj, o, 64);
-            fyj += __shfl_xor(fyj, o, 64);
-            fzj += __shfl_xor(fzj, o, 64);
-        }
-        if (ia == 0) {
-            facc_add(&FX[sj], fxj);
-            facc_add(&FY[sj], fyj);
-            facc_add(&FZ[sj], fzj);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        if (k < ncl) {
-            real fx = fxi[k], fy = fyi[k], fz = fzi[k];
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1) {
-                fx += __shfl_xor(fx, o, 64);
-                fy += __shfl_xor(fy, o, 64);
-                fz += __shfl_xor(fz, o, 64);
-            }
-            if (ja == 0) {
-                const int slot = (fc + k) * 8 + ia;
-                facc_add(&FX[slot], fx);
-                facc_add(&FY[slot], fy);
-                facc_add(&FZ[slot], fz);
+        for (int k = 0; k < 8; ++k) {
+            if (k < ncl) {
+                const real fx = sum_group8(fxi[k]), fy = sum_group8(fyi[k]), fz = sum_group8(fzi[k]);
+                if (ja == 0) {
+                    const int slot = (fc + k) * 8 + ia;
+                    facc_add(&FX[slot], fx);
+                    facc_add(&FY[slot], fy);
+                    facc_add(&FZ[slot], fz);
+                }
             }
         }
-    }
-    if constexpr (ENERGY) {
-        ec_acc = wave_sum(ec_acc);
-        el_acc = wave_sum(el_acc);
-        if (lane == 0) {
-            energy_add(&energy[s * E_NTERMS + E_COUL], ec_acc);
-            energy_add(&energy[s * E_NTERMS + E_LJ], el_acc);
+        if constexpr (ENERGY) {
+            ec_acc = wave_sum(ec_acc);
+            el_acc = wave_sum(el_acc);
+            if (lane == 0) {
+                energy_add(&energy[s * E_NTERMS + E_COUL], ec_acc);
+                energy_add(&energy[s * E_NTERMS + E_LJ], el_acc);
+            }
         }
+        wave_lds_sync();   // LDS of this wave is reused by its next chunk
     }
 }
 
@@ -256,10 +338,7 @@ __global__ void exclusion_kernel(const SysDev* __restrict__ sd, const int* __res
             }
         }
     }
-    if constexpr (ENERGY) {
-        // pairs of one block may belong to several systems: add per thread when mixed systems
-        if (s >= 0) energy_add(&energy[s * E_NTERMS + E_EXCL], eacc);
-    }
+    if constexpr (ENERGY) wave_energy_add(energy, s < 0 ? 0 : s, E_EXCL, eacc, s >= 0);
 }
 
 // Explicit pair set (atom ids, first > second) with the FP64 test of the pair kernel.
@@ -302,15 +381,16 @@ __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, cons
 
 template <class P>
 void launch_pair(Batch<P>& B, bool energy) {
-    if (B.n_super_cap == 0) return;
+    if (B.n_chunk_cap == 0) return;
+    pair_prefix_kernel<<<1, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_prefix);
+    // persistent grid: ~24 resident wavefronts per CU on a 256-CU part
+    const unsigned blocks = (unsigned)std::max(1, std::min((B.n_chunk_cap + kPairWaves - 1) / kPairWaves, 1536));
     if (energy)
-        pair_kernel<P, true><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.list,
-                                                                 B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz,
-                                                                 B.energy, B.n_super_cap);
+        pair_kernel<P, true><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.chunks, B.list,
+                                                           B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);
     else
-        pair_kernel<P, false><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item,
-                                                                  B.list, B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz,
-                                                                  B.energy, B.n_super_cap);
+        pair_kernel<P, false><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.chunks, B.list,
+                                                            B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);
 }
 
 template <class P>

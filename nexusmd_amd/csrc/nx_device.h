@@ -44,6 +44,7 @@ struct SysDev {
     int slot_off, slot_cap;
     int col_off, ncx, ncy, col_cap;   // columns: ncx*ncy
     int super_off, super_cap;
+    int chunk_off, chunk_cap;    // pair-kernel work items (list chunks)
     int list_off, list_cap;
     int emask_off, emask_cap;
     double colw[2];              // column widths
@@ -52,13 +53,18 @@ struct SysDev {
     int excl_off, n_excl;        // excluded pair list
     int bond_off, n_bonds, angle_off, n_angles, tors_off, n_tors;
     int tw_off[3];               // twiddle-table offsets per axis
+    int sp_block[3];             // spread tiling: columns per block in x, y; z slabs (0 = direct atomics)
+    double sp_margin;            // distance an atom may move between rebuilds
 };
 
 // Per-system counters written by the device.
 struct SysCount {
     int n_slots, n_clusters, n_super, list_count;
     int emask_count, rebuild, overflow, rebuilds;
+    int n_chunks, pad0, pad1, pad2;
 };
+
+constexpr int kChunkEntries = 32;       // pair-list entries per pair-kernel wavefront
 
 // ------------------------------------------------------------ precision
 
@@ -135,21 +141,79 @@ __device__ __forceinline__ void block_energy_add(long long* e, double v) {
     __syncthreads();
 }
 
-// Reference minimum image: one +-L correction (PeriodicBoundaryCondition.cpp:93-104).
+// Adds v of the valid lanes to energy[s * E_NTERMS + term] with one atomic per
+// wavefront when all valid lanes belong to one system.  Every lane must call it.
+__device__ __forceinline__ void wave_energy_add(long long* energy, int s, int term, double v, bool valid) {
+    const unsigned long long m = __ballot(valid);
+    if (m == 0ull) return;
+    const int lead = __ffsll((long long)m) - 1;
+    const int s0 = __shfl(s, lead, 64);
+    if (__all(!valid || s == s0)) {
+        v = wave_sum(valid ? v : 0.0);
+        if ((int)(threadIdx.x & 63) == lead) energy_add(&energy[s0 * E_NTERMS + term], v);
+    } else if (valid) {
+        energy_add(&energy[s * E_NTERMS + term], v);
+    }
+}
+
+// Reference minimum image: one +-L correction (PeriodicBoundaryCondition.cpp:93-104),
+// written with selects so that it compiles without branches.
 __device__ __forceinline__ double min_image1(double d, double L) {
-    if (d > 0.5 * L) d -= L;
-    else if (d < -0.5 * L) d += L;
-    return d;
+    const double h = 0.5 * L;
+    const double up = d - L, dn = d + L;
+    return d > h ? up : (d < -h ? dn : d);
 }
 __device__ __forceinline__ float min_image1(float d, float L) {
-    if (d > 0.5f * L) d -= L;
-    else if (d < -0.5f * L) d += L;
-    return d;
+    const float h = 0.5f * L;
+    const float up = d - L, dn = d + L;
+    return d > h ? up : (d < -h ? dn : d);
 }
 
 // r^2 without contraction, left-to-right as Coords3D::dot (Coords3D.h:116-124).
 __device__ __forceinline__ double dot_rn(double x, double y, double z) {
     return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+// Cross-lane sums on a 64-wide wavefront without LDS: DPP row rotate for
+// lane^8, permlane16/32 swaps for lane^16 and lane^32, DPP quad permutes and
+// the half-row mirror for lane^1, lane^2 and the 8-lane halves.
+__device__ __forceinline__ int dpp_ror8(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_xor1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_xor2(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_hmirror(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false); }
+
+// sum over lanes {l, l^8, l^16, ..., l^56} (lanes sharing l & 7)
+__device__ __forceinline__ float sum_stride8(float v) {
+    v += __int_as_float(dpp_ror8(__float_as_int(v)));
+    auto p = __builtin_amdgcn_permlane16_swap(__float_as_int(v), __float_as_int(v), false, false);
+    v = __int_as_float(p[0]) + __int_as_float(p[1]);
+    auto q = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+    return __int_as_float(q[0]) + __int_as_float(q[1]);
+}
+__device__ __forceinline__ double sum_stride8(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    v += __hiloint2double(dpp_ror8(hi), dpp_ror8(lo));
+    lo = __double2loint(v); hi = __double2hiint(v);
+    auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    v = __hiloint2double(ph[0], pl[0]) + __hiloint2double(ph[1], pl[1]);
+    lo = __double2loint(v); hi = __double2hiint(v);
+    auto ql = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    auto qh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double(qh[0], ql[0]) + __hiloint2double(qh[1], ql[1]);
+}
+// sum over the 8 lanes l & ~7 .. l | 7
+__device__ __forceinline__ float sum_group8(float v) {
+    v += __int_as_float(dpp_xor1(__float_as_int(v)));
+    v += __int_as_float(dpp_xor2(__float_as_int(v)));
+    v += __int_as_float(dpp_hmirror(__float_as_int(v)));
+    return v;
+}
+__device__ __forceinline__ double sum_group8(double v) {
+    v += __hiloint2double(dpp_xor1(__double2hiint(v)), dpp_xor1(__double2loint(v)));
+    v += __hiloint2double(dpp_xor2(__double2hiint(v)), dpp_xor2(__double2loint(v)));
+    v += __hiloint2double(dpp_hmirror(__double2hiint(v)), dpp_hmirror(__double2loint(v)));
+    return v;
 }
 
 // XCD-aware block order: consecutive work items land on one XCD (8 XCDs,

@@ -52,6 +52,10 @@ struct Batch {
     float4 *cl_ctr = nullptr, *cl_half = nullptr;
     int* super_sys = nullptr;
     int4* super_item = nullptr;
+    int* chunk_sys = nullptr;
+    int4* chunks = nullptr;         // (first cluster, n clusters, first entry, end entry)
+    int* chunk_prefix = nullptr;    // [nsys + 1] running total of chunks over systems
+    int n_chunk_cap = 0;
     uint2* list = nullptr;
     unsigned long long* emask = nullptr;
     int *excl_off = nullptr, *excl_idx = nullptr;
@@ -76,6 +80,9 @@ struct Batch {
     real* eta = nullptr;            // influence function on the half spectrum
     real2* twiddle = nullptr;       // per system, per axis exp(-2 pi i t / K)
     long long rgrid_total = 0, cgrid_total = 0, tw_total = 0;
+    int4* spread_tiles = nullptr;   // (sys, column block x, column block y, z slab)
+    int n_spread_tiles = 0, n_spread_direct_slots = 0;
+    float* slot_zkey = nullptr;     // build-time z of each slot (sorted within a column)
     int4* fftz_items = nullptr;     // (sys, first line, n lines, 0)
     int4* ffty_items = nullptr;     // (sys, x, kz0, 0)
     int4* fftx_items = nullptr;     // (sys, y, kz0, 0)

@@ -69,6 +69,16 @@ __device__ __forceinline__ void grid_add(double* g, double v) { unsafeAtomicAdd(
 __device__ __forceinline__ void grid_add(long long* g, float v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(to_fixed(v)));
 }
+// LDS accumulation: fixed point (mixed, exact and order-free) or FP64
+__device__ __forceinline__ void grid_add_lds(double* g, double v) { atomicAdd(g, v); }
+__device__ __forceinline__ void grid_add_lds(long long* g, float v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(to_fixed(v)));
+}
+// adding an already accumulated value
+__device__ __forceinline__ void grid_add_acc(double* g, double v) { unsafeAtomicAdd(g, v); }
+__device__ __forceinline__ void grid_add_acc(long long* g, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(v));
+}
 
 // One thread per (slot, ix): 25 grid updates each.
 template <class P>
@@ -84,7 +94,7 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
     const SysDev& d = sd[s];
     if (slot - d.slot_off >= sc[s].n_slots) return;
     if (atom_of_slot[slot] < 0) return;
-    if (d.grid[0] == 0) return;
+    if (d.grid[0] == 0 || d.sp_block[0] > 0) return;
     const typename P::real4 p = pos[slot];
     const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
     int i0, i1, i2;
@@ -108,6 +118,136 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
             const int zi = (i2 + iz) % Kz;
             grid_add(&g[(long long)yi * Kz + zi], qxy * tz[iz]);
         }
+    }
+}
+
+// LDS-tiled spreading.  One workgroup per (system, column block, z slab):
+// the atoms binned into those columns at the last neighbour-list build whose
+// build-time z lies in the slab.  Their stencils are accumulated in an LDS copy
+// of the grid region they can reach (block + slab, widened by the distance an
+// atom may move between rebuilds, + 4 points for the order-5 stencil), then
+// the region is added to the global grid once, z-contiguous.  An atom outside
+// the region (never, while the rebuild criterion holds) adds directly.
+constexpr int kSpreadLdsPoints = 7680;   // 60 KB of int64 / double
+
+template <class P>
+__global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict__ sd,
+                                                        const SysCount* __restrict__ sc,
+                                                        const int4* __restrict__ tiles,
+                                                        const int* __restrict__ col_count,
+                                                        const int* __restrict__ col_slot,
+                                                        const float* __restrict__ slot_zkey,
+                                                        const typename P::real4* __restrict__ pos,
+                                                        typename P::facc* rgrid) {
+    using real = typename P::real;
+    using acc = typename P::facc;
+    __shared__ acc g[kSpreadLdsPoints];
+    const int4 t = tiles[blockIdx.x];          // (sys, bx, by, bz)
+    const int s = t.x;
+    const SysDev& d = sd[s];
+    if (sc[s].n_slots == 0) return;
+    const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
+    const int BX = d.sp_block[0], BY = d.sp_block[1], NZ = d.sp_block[2];
+    const int cx0 = t.y * BX, cx1 = min(d.ncx, cx0 + BX);
+    const int cy0 = t.z * BY, cy1 = min(d.ncy, cy0 + BY);
+    const double zs = d.L[2] / NZ;
+    const double z0 = t.w * zs, z1 = (t.w + 1 == NZ) ? d.L[2] + 1.0 : (t.w + 1) * zs;
+    const double zlo = t.w * zs;
+    const double m = d.sp_margin;
+    // region origin and extent per axis (full axis when the block spans it)
+    int lo[3], n[3];
+    {
+        const double a0[3] = {cx0 * d.colw[0], cy0 * d.colw[1], zlo};
+        const double a1[3] = {cx1 * d.colw[0], cy1 * d.colw[1], (t.w + 1) * zs};
+        const bool full[3] = {cx0 == 0 && cx1 == d.ncx, cy0 == 0 && cy1 == d.ncy, NZ == 1};
+        const int K[3] = {Kx, Ky, Kz};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double h = d.L[k] / K[k];
+            int l = (int)floor((a0[k] - m) / h);
+            int u = (int)floor((a1[k] + m) / h) + 4;
+            if (full[k] || u - l + 1 >= K[k]) { l = 0; u = K[k] - 1; }
+            lo[k] = l;
+            n[k] = u - l + 1;
+        }
+    }
+    const bool fx = (n[0] == Kx && lo[0] == 0), fy = (n[1] == Ky && lo[1] == 0), fz = (n[2] == Kz && lo[2] == 0);
+    const int npts = n[0] * n[1] * n[2];
+    if (npts > kSpreadLdsPoints) return;       // host guarantees the fit
+    for (int i = threadIdx.x; i < npts; i += blockDim.x) g[i] = acc(0);
+    __syncthreads();
+    // atoms of the block's columns in the slab: 5 threads per atom (one per x plane)
+    for (int c = cx0; c < cx1; ++c) {
+        for (int cyy = cy0; cyy < cy1; ++cyy) {
+            const int col = d.col_off + c * d.ncy + cyy;
+            const int first = col_slot[col], cnt = col_count[col];
+            // slots are sorted by build-time z: binary search the slab
+            int a = 0, b = cnt;
+            while (a < b) { const int mid = (a + b) >> 1; if ((double)slot_zkey[first + mid] < z0) a = mid + 1; else b = mid; }
+            int e = a, f = cnt;
+            while (e < f) { const int mid = (e + f) >> 1; if ((double)slot_zkey[first + mid] < z1) e = mid + 1; else f = mid; }
+            const int nat = e - a;
+            for (int w = threadIdx.x; w < nat * 5; w += blockDim.x) {
+                const int slot = first + a + w / 5, ix = w % 5;
+                const typename P::real4 p = pos[slot];
+                int i0, i1, i2;
+                real f0, f1, f2;
+                grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
+                grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
+                grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
+                real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
+                bspline5<real>(f0, tx, dtx);
+                bspline5<real>(f1, ty, dty);
+                bspline5<real>(f2, tz, dtz);
+                int u0 = fx ? i0 : i0 - lo[0];
+                int u1 = fy ? i1 : i1 - lo[1];
+                int u2 = fz ? i2 : i2 - lo[2];
+                if (!fx) { if (u0 < 0) u0 += Kx; else if (u0 >= Kx) u0 -= Kx; }
+                if (!fy) { if (u1 < 0) u1 += Ky; else if (u1 >= Ky) u1 -= Ky; }
+                if (!fz) { if (u2 < 0) u2 += Kz; else if (u2 >= Kz) u2 -= Kz; }
+                const bool inside = (fx || (u0 >= 0 && u0 + 4 < n[0])) && (fy || (u1 >= 0 && u1 + 4 < n[1])) &&
+                                    (fz || (u2 >= 0 && u2 + 4 < n[2]));
+                const real qx = p.w * tx[ix];
+                if (inside) {
+                    int x = u0 + ix;
+                    if (fx && x >= Kx) x -= Kx;
+#pragma unroll
+                    for (int iy = 0; iy < 5; ++iy) {
+                        int y = u1 + iy;
+                        if (fy && y >= Ky) y -= Ky;
+                        const real qxy = qx * ty[iy];
+#pragma unroll
+                        for (int iz = 0; iz < 5; ++iz) {
+                            int z = u2 + iz;
+                            if (fz && z >= Kz) z -= Kz;
+                            grid_add_lds(&g[(x * n[1] + y) * n[2] + z], qxy * tz[iz]);
+                        }
+                    }
+                } else {
+                    const int xi = (i0 + ix) % Kx;
+                    acc* gg = rgrid + d.rgrid_off + (long long)xi * Ky * Kz;
+#pragma unroll
+                    for (int iy = 0; iy < 5; ++iy) {
+                        const int yi = (i1 + iy) % Ky;
+                        const real qxy = qx * ty[iy];
+#pragma unroll
+                        for (int iz = 0; iz < 5; ++iz) grid_add(&gg[(long long)yi * Kz + (i2 + iz) % Kz], qxy * tz[iz]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // flush the region (z fastest: contiguous global addresses)
+    for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+        const acc v = g[i];
+        if (v == acc(0)) continue;
+        const int z = i % n[2], y = (i / n[2]) % n[1], x = i / (n[1] * n[2]);
+        int gx = lo[0] + x, gy = lo[1] + y, gz = lo[2] + z;
+        gx = gx < 0 ? gx + Kx : (gx >= Kx ? gx - Kx : gx);
+        gy = gy < 0 ? gy + Ky : (gy >= Ky ? gy - Ky : gy);
+        gz = gz < 0 ? gz + Kz : (gz >= Kz ? gz - Kz : gz);
+        grid_add_acc(&rgrid[d.rgrid_off + ((long long)gx * Ky + gy) * Kz + gz], v);
     }
 }
 
@@ -455,10 +595,16 @@ void launch_pme_spread(Batch<P>& B) {
     if (!B.pme_any) return;
     const int T = 256;
     fill_zero<typename P::facc><<<(unsigned)((B.rgrid_total + T - 1) / T), T, 0, B.stream>>>(B.rgrid, B.rgrid_total);
-    const long long nthr = (long long)B.n_slots_cap * 5;
-    pme_spread_kernel<P><<<(unsigned)((nthr + T - 1) / T), T, 0, B.stream>>>(B.d_sys, B.d_cnt, B.slot_sys,
-                                                                             B.atom_of_slot, B.pos, B.rgrid,
-                                                                             B.n_slots_cap);
+    if (B.n_spread_tiles > 0)
+        pme_spread_tiled<P><<<B.n_spread_tiles, 256, 0, B.stream>>>(B.d_sys, B.d_cnt, B.spread_tiles, B.col_count,
+                                                                    B.col_slot, B.slot_zkey, B.pos, B.rgrid);
+    if (B.n_spread_direct_slots > 0) {
+        // systems whose region does not fit the LDS tile: per-atom global atomics
+        const long long nthr = (long long)B.n_slots_cap * 5;
+        pme_spread_kernel<P><<<(unsigned)((nthr + T - 1) / T), T, 0, B.stream>>>(B.d_sys, B.d_cnt, B.slot_sys,
+                                                                                 B.atom_of_slot, B.pos, B.rgrid,
+                                                                                 B.n_slots_cap);
+    }
 }
 
 template <class P>
