@@ -274,3 +274,64 @@ def pme_recip(sys, grid, pos=None):
     f = np.zeros_like(pos)
     e = lib().nxo_pme_recip(pos.shape[0], _d(pos), _d(q), C.byref(box), g, alpha(sys.cutoff, sys.ewald_tol), _d(f))
     return f, e
+
+
+def box_from_positions(pos):
+    pos = np.ascontiguousarray(pos, np.float64)
+    b = Box()
+    lib().nxo_box_from_positions(pos.shape[0], _d(pos), C.byref(b))
+    return np.array(b.lb[:]), np.array(b.ub[:]), np.array(b.L[:])
+
+
+def grid_rule(lb, ub):
+    b = Box()
+    for k in range(3):
+        b.lb[k], b.ub[k] = float(lb[k]), float(ub[k])
+        b.L[k] = abs(float(ub[k]) - float(lb[k]))
+    g = (C.c_int * 3)()
+    lib().nxo_grid_rule(C.byref(b), g)
+    return np.array(g[:], np.int32)
+
+
+def bspline_moduli(K, order=5):
+    m = np.zeros(int(K))
+    lib().nxo_bspline_moduli(int(K), int(order), _d(m))
+    return m
+
+
+def bspline_theta(frac, order=5):
+    th, dth = np.zeros(order), np.zeros(order)
+    lib().nxo_bspline_theta(float(frac), int(order), _d(th), _d(dth))
+    return th, dth
+
+
+def particle_index(x, sys, grid):
+    x = np.ascontiguousarray(x, np.float64)
+    box = box_of(sys)
+    g = (C.c_int * 3)(*[int(k) for k in grid])
+    idx = (C.c_int * 3)()
+    fr = np.zeros(3)
+    lib().nxo_particle_index(_d(x), C.byref(box), g, idx, _d(fr))
+    return np.array(idx[:], np.int32), fr
+
+
+def pme_spread(sys, grid, pos=None, q=None):
+    pos = np.ascontiguousarray(sys.pos if pos is None else pos, np.float64)
+    q = np.ascontiguousarray(sys.q if q is None else q, np.float64)
+    g = (C.c_int * 3)(*[int(k) for k in grid])
+    c = np.zeros(tuple(int(k) for k in grid), np.complex128)
+    box = box_of(sys)
+    lib().nxo_pme_spread(pos.shape[0], _d(pos), _d(q), C.byref(box), g, c.ctypes.data_as(D))
+    return c
+
+
+def nonbonded_real(prep: Prepared, pos=None):
+    """Self, direct and exclusion terms only (mask NONBONDED minus the reciprocal part)."""
+    sys = prep.sys
+    pos = np.ascontiguousarray(sys.pos if pos is None else pos, np.float64)
+    f_all, e = compute_forces(prep, pos, NONBONDED)
+    grid = prep.c.grid[:]
+    if min(grid) <= 0:
+        grid = grid_rule(sys.lb, sys.ub)
+    f_rec, _ = pme_recip(sys, grid, pos)
+    return f_all - f_rec, e

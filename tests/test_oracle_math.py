@@ -1,0 +1,273 @@
+"""The oracle against what the mathematics fixes, independent of the reference.
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong sign or index, a transposed operand) fails it: forces against finite
+differences of the energies, Newton's third law, the FFT against numpy and a
+brute-force DFT, the B-spline partition of unity and exact knot values, the
+PME reciprocal sum against a plain Ewald k-space sum, the rock-salt Madelung
+constant, pair sets against brute-force enumeration, exclusions against a
+graph search, and the leapfrog update on a free particle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nexusmd_amd import systems as S
+from oracle import pyoracle as O
+
+from _util import bare_system, rel_rms
+
+KE = 138.93545764446428693393
+
+
+@pytest.fixture(scope="module")
+def chain():
+    s = S.bead_chain(n=120, L=2.4, grid=(24, 24, 24), seed=31)
+    rng = np.random.default_rng(31)
+    s.pos = (s.pos + rng.normal(0.0, 0.01, s.pos.shape)) % s.L
+    return s
+
+
+def _energy(prep, pos, mask, key):
+    _, e = O.compute_forces(prep, pos=pos, mask=mask)
+    return e[key]
+
+
+@pytest.mark.parametrize("term,mask", [("bond", O.BOND), ("angle", O.ANGLE), ("torsion", O.TORSION)])
+def test_bonded_forces_are_minus_gradient(chain, term, mask):
+    prep = O.Prepared(chain)
+    f, _ = O.compute_forces(prep, mask=mask)
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for i in rng.choice(chain.n, 12, replace=False):
+        for d in range(3):
+            p = chain.pos.copy()
+            p[i, d] += h
+            ep = _energy(prep, p, mask, term)
+            p[i, d] -= 2 * h
+            em = _energy(prep, p, mask, term)
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - f[i, d]) <= 1e-4 * max(1.0, abs(f[i, d])), (term, i, d, fd, f[i, d])
+
+
+def test_direct_forces_are_minus_gradient():
+    rng = np.random.default_rng(5)
+    s = bare_system(rng.uniform(0, 2.2, (40, 3)), [0, 0, 0], [2.2, 2.2, 2.2], q=rng.normal(0, 0.5, 40),
+                    sig=rng.uniform(0.25, 0.35, 40), eps=rng.uniform(0.1, 1.0, 40), cutoff=1.0)
+    # keep particles apart so the LJ terms stay moderate
+    s.pos = np.array([[x, y, z] for x in (0.2, 0.75, 1.3, 1.85) for y in (0.2, 0.75, 1.3, 1.85)
+                      for z in (0.2, 1.0, 1.7)])[:40] + rng.normal(0, 0.03, (40, 3))
+    prep = O.Prepared(s, excl=(np.zeros(s.n + 1, np.int32), np.zeros(0, np.int32)))
+    f, _ = O.nonbonded_real(prep)
+    h = 1e-6
+    for i in (0, 7, 19, 33):
+        for d in range(3):
+            p = s.pos.copy()
+            p[i, d] += h
+            _, ep = O.nonbonded_real(prep, pos=p)
+            p[i, d] -= 2 * h
+            _, em = O.nonbonded_real(prep, pos=p)
+            de = (ep["coulomb"] + ep["lj"]) - (em["coulomb"] + em["lj"])
+            # pairs crossing the cutoff inside +-h would show as jumps; none do here
+            assert abs(-de / (2 * h) - f[i, d]) <= 1e-4 * max(1.0, abs(f[i, d]))
+
+
+@pytest.mark.parametrize("mask", [O.BOND, O.ANGLE, O.TORSION])
+def test_bonded_newton_third_law(chain, mask):
+    f, _ = O.compute_forces(O.Prepared(chain), mask=mask)
+    assert np.abs(f.sum(0)).max() <= 1e-9 * max(1.0, np.abs(f).max())
+
+
+def test_real_space_newton_third_law(chain):
+    f, _ = O.nonbonded_real(O.Prepared(chain))
+    assert np.abs(f.sum(0)).max() <= 1e-9 * np.abs(f).max()
+
+
+def test_translation_invariance(chain):
+    prep = O.Prepared(chain)
+    f0, e0 = O.compute_forces(prep, mask=O.BOND | O.ANGLE | O.TORSION)
+    f1, e1 = O.compute_forces(prep, pos=(chain.pos + np.array([0.31, -0.17, 0.05])) % chain.L,
+                              mask=O.BOND | O.ANGLE | O.TORSION)
+    assert rel_rms(f1, f0) <= 1e-10
+    assert abs(e1["potential"] - e0["potential"]) <= 1e-9 * abs(e0["potential"])
+
+
+# ------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (8, 6, 10), (12, 9, 5), (16, 30, 24), (40, 40, 40), (7, 3, 11)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_matches_numpy(shape, sign):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    got = O.fft3d(a, sign)
+    ref = np.fft.fftn(a) if sign < 0 else np.fft.ifftn(a) * a.size
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_fft_matches_brute_dft():
+    rng = np.random.default_rng(9)
+    shape = (3, 4, 5)
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    idx = [np.arange(k) for k in shape]
+    out = np.zeros(shape, complex)
+    for kx in range(3):
+        for ky in range(4):
+            for kz in range(5):
+                ph = np.exp(-2j * np.pi * (kx * idx[0][:, None, None] / 3 + ky * idx[1][None, :, None] / 4
+                                           + kz * idx[2][None, None, :] / 5))
+                out[kx, ky, kz] = np.sum(a * ph)
+    assert np.abs(O.fft3d(a, -1) - out).max() <= 1e-12 * np.abs(out).max()
+
+
+# -------------------------------------------------------------- B-splines
+
+def test_bspline_partition_of_unity():
+    for frac in np.linspace(0.0, 0.999, 37):
+        th, dth = O.bspline_theta(frac)
+        assert abs(th.sum() - 1.0) <= 1e-14
+        assert abs(dth.sum()) <= 1e-14
+        assert np.all(th >= 0)
+
+
+def test_bspline_knot_values():
+    # M5 at the integer knots: 1/24, 11/24, 11/24, 1/24 (frac = 0 gives theta = M5(k+1) reversed)
+    th, _ = O.bspline_theta(0.0)
+    np.testing.assert_allclose(sorted(th[th > 0]), sorted([1 / 24, 11 / 24, 11 / 24, 1 / 24]), rtol=0,
+                               atol=1e-15)
+
+
+def test_bspline_moduli_properties():
+    for K in (16, 24, 30, 40, 45):
+        m = O.bspline_moduli(K)
+        assert abs(m[0] - 1.0) <= 1e-14          # sum of M5 at the knots is 1
+        np.testing.assert_allclose(m[1:], m[1:][::-1], rtol=1e-12)   # |b(m)|^2 = |b(K - m)|^2
+        assert np.all(m > 0)
+    # even K: M5's transform vanishes at K/2; the reference replaces values below
+    # 1e-7 by the neighbours' mean (NonbondedForce.cpp:137-141)
+    m = O.bspline_moduli(40)
+    assert abs(m[20] - 0.5 * (m[19] + m[21])) <= 1e-15
+
+
+# -------------------------------------------------------------- Ewald / PME
+
+def _ewald_recip(pos, q, L, alpha, kmax):
+    """Plain Ewald k-space energy and forces, E = (2 pi ke / V) sum_k exp(-k^2/4a^2)/k^2 |S(k)|^2 over k != 0
+    (half space doubled)."""
+    V = L ** 3
+    e = 0.0
+    f = np.zeros_like(pos)
+    rng = range(-kmax, kmax + 1)
+    for mx in rng:
+        for my in rng:
+            for mz in rng:
+                if (mx, my, mz) == (0, 0, 0):
+                    continue
+                k = 2 * np.pi * np.array([mx, my, mz]) / L
+                k2 = k @ k
+                w = math.exp(-k2 / (4 * alpha * alpha)) / k2
+                ph = pos @ k
+                c, s = q * np.cos(ph), q * np.sin(ph)
+                C, Sm = c.sum(), s.sum()
+                e += w * (C * C + Sm * Sm)
+                # dE/dr_i = (2 pi ke / V) w 2 q_i (-sin(k r_i) C + cos(k r_i) S) k
+                g = 2 * (-s * C + c * Sm)
+                f -= (w * g)[:, None] * k[None, :]
+    return 2 * np.pi * KE / V * e, 2 * np.pi * KE / V * f
+
+
+def test_pme_recip_matches_ewald_sum():
+    rng = np.random.default_rng(17)
+    n, L = 24, 2.0
+    q = rng.normal(0, 0.5, n)
+    q -= q.mean()
+    s = bare_system(rng.uniform(0, L, (n, 3)), [0, 0, 0], [L, L, L], q=q, cutoff=0.9, ewald_tol=5e-4)
+    alpha = O.alpha(s.cutoff, s.ewald_tol)
+    f_pme, e_pme = O.pme_recip(s, (48, 48, 48))
+    e_ew, f_ew = _ewald_recip(s.pos, q, L, alpha, 9)
+    assert abs(e_pme - e_ew) <= 2e-5 * abs(e_ew)
+    assert rel_rms(f_pme, f_ew) <= 1e-4
+
+
+def test_madelung_rock_salt():
+    """Coulomb energy of a NaCl crystal per ion pair is -M ke / r0 with M = 1.747565."""
+    a, cells = 0.5, 4
+    pts, q = [], []
+    for i in range(2 * cells):
+        for j in range(2 * cells):
+            for k in range(2 * cells):
+                pts.append((i * a / 2, j * a / 2, k * a / 2))
+                q.append(1.0 if (i + j + k) % 2 == 0 else -1.0)
+    L = cells * a
+    s = bare_system(np.array(pts) + 0.01, [0, 0, 0], [L, L, L], q=np.array(q), cutoff=0.95, ewald_tol=1e-7,
+                    grid=(64, 64, 64))
+    prep = O.Prepared(s, excl=(np.zeros(s.n + 1, np.int32), np.zeros(0, np.int32)))
+    _, e = O.compute_forces(prep, mask=O.NONBONDED)
+    coulomb = e["coulomb"] + e["recip"] + e["self"] - e["exclusion"]
+    madelung = -coulomb / (s.n / 2) * (a / 2) / KE
+    assert abs(madelung - 1.747565) <= 2e-5
+
+
+# ------------------------------------------------------ pairs, exclusions
+
+@pytest.mark.parametrize("seed,L,n", [(1, 3.0, 300), (2, 2.1, 200), (3, 1.7, 90), (4, 4.3, 500)])
+def test_voxel_pairs_equal_brute_force(seed, L, n):
+    rng = np.random.default_rng(seed)
+    s = bare_system(rng.uniform(0, L, (n, 3)), [0, 0, 0], [L, L, L])
+    none = (np.zeros(n + 1, np.int32), np.zeros(0, np.int32))
+    v = O.pairs(s, none, method="voxel")
+    b = O.pairs(s, none, method="brute")
+    assert set(map(tuple, v.tolist())) == set(map(tuple, b.tolist()))
+    # and the brute force is what a direct numpy enumeration with single-correction min image gives
+    d = s.pos[:, None, :] - s.pos[None, :, :]
+    d = np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
+    r2 = (d * d).sum(-1)
+    i, j = np.nonzero(np.tril(r2 < 1.0, -1))
+    assert set(zip(i.tolist(), j.tolist())) == set(map(tuple, b.tolist()))
+
+
+def test_exclusions_are_graph_distance_three(chain):
+    off, idx = O.exclusions(chain)
+    ref_off, ref_idx = S.exclusions_csr(chain.n, chain.bond_idx)
+    np.testing.assert_array_equal(off, ref_off)
+    for i in range(chain.n):
+        assert sorted(idx[off[i]:off[i + 1]]) == sorted(ref_idx[ref_off[i]:ref_off[i + 1]])
+
+
+def test_exclusions_branched():
+    # a star: centre 0 bonded to 1..4, and a tail 4-5-6-7-8
+    bonds = np.array([[0, 1], [0, 2], [0, 3], [0, 4], [4, 5], [5, 6], [6, 7], [7, 8]], np.int32)
+    s = bare_system(np.zeros((9, 3)), [0, 0, 0], [3, 3, 3])
+    s.bond_idx = bonds
+    off, idx = O.exclusions(s)
+    ex = {i: set(idx[off[i]:off[i + 1]].tolist()) for i in range(9)}
+    assert ex[1] == {0, 2, 3, 4, 5}
+    assert ex[0] == {1, 2, 3, 4, 5, 6}
+    assert ex[8] == {7, 6, 5}
+    assert all(i in ex[j] for i in range(9) for j in ex[i])
+
+
+# ------------------------------------------------------------ integrator
+
+def test_leapfrog_free_particle_and_wrap():
+    s = bare_system(np.array([[1.0, 1.0, 1.0], [0.01, 2.99, 1.5]]), [0, 0, 0], [3, 3, 3],
+                    mass=np.array([2.0, 4.0]))
+    v0 = np.array([[1.0, -2.0, 0.5], [-3.0, 3.0, 0.0]])
+    f = np.array([[4.0, 0.0, -2.0], [0.0, 0.0, 8.0]])
+    dt = 0.01
+    pos, vel = O.leapfrog(s, s.pos, v0, f, dt)
+    v1 = v0 + f / s.mass[:, None] * dt
+    np.testing.assert_allclose(vel, v1, rtol=1e-15)
+    x1 = s.pos + v1 * dt
+    x1 = np.where(x1 < 0, x1 + 3.0, np.where(x1 > 3.0, x1 - 3.0, x1))
+    np.testing.assert_allclose(pos, x1, rtol=1e-14)
+
+
+def test_kinetic_half_step_shift():
+    s = bare_system(np.zeros((2, 3)), [0, 0, 0], [3, 3, 3], mass=np.array([2.0, 3.0]))
+    v = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0]])
+    f = np.array([[2.0, 0.0, 0.0], [0.0, -3.0, 0.0]])
+    dt = 0.1
+    ke = O.kinetic(s, v, f, dt)
+    u = v + f / s.mass[:, None] * dt / 2
+    assert abs(ke - 0.5 * float((s.mass[:, None] * u * u).sum())) <= 1e-14
