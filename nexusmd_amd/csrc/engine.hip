@@ -514,20 +514,28 @@ class Engine : public EngineBase {
         HIPCHK(hipMemsetAsync(B.slot_of_atom, 0, sizeof(int) * B.n_atoms, st));
         HIPCHK(hipMemsetAsync(B.super_item, 0, sizeof(int4) * B.n_super_cap, st));
         // positions and velocities
-        std::vector<double> x, y, z, vx, vy, vz;
+        // The first build runs on each system's carried-over build positions
+        // when it has them and no rebuild was due, else on its current positions.
+        std::vector<double> x, y, z, vx, vy, vz, bx, by, bz;
+        bool carried = false;
         for (int s = 0; s < ns; ++s) {
             HostSystem& h = *hs_[s];
+            const bool use_b = !h.build_pos.empty() && !h.rebuild_pending;
+            const double* b = use_b ? h.build_pos.data() : h.pos.data();
+            carried |= use_b;
             for (int i = 0; i < h.n; ++i) {
                 x.push_back(h.pos[3 * i]); y.push_back(h.pos[3 * i + 1]); z.push_back(h.pos[3 * i + 2]);
                 vx.push_back(h.vel[3 * i]); vy.push_back(h.vel[3 * i + 1]); vz.push_back(h.vel[3 * i + 2]);
+                bx.push_back(b[3 * i]); by.push_back(b[3 * i + 1]); bz.push_back(b[3 * i + 2]);
             }
         }
-        upload(B.X, x, st); upload(B.Y, y, st); upload(B.Z, z, st);
+        upload(B.X, bx, st); upload(B.Y, by, st); upload(B.Z, bz, st);
         upload(B.VX, vx, st); upload(B.VY, vy, st); upload(B.VZ, vz, st);
         request_rebuild();
         HIPCHK(hipStreamSynchronize(st));
         // first build; re-lay out with larger list capacities until the lists fit
         launch_nlist<P>(B);
+        if (carried) { upload(B.X, x, st); upload(B.Y, y, st); upload(B.Z, z, st); }
         HIPCHK(hipStreamSynchronize(st));
         std::vector<SysCount> c(ns);
         copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * ns, hipMemcpyDeviceToHost);
@@ -863,11 +871,20 @@ class Engine : public EngineBase {
             for (size_t a = 0; a < n; ++a) dst[k][3 * a + (k % 3)] = h[a];
         }
     }
-    size_t bytes() const override {
-        size_t tot = 0;
-        (void)tot;
-        return footprint_bytes(B);
+    void device_list_state(int s, double* build_pos, int* rebuild_due) override {
+        const SysDev& d = B.sys[s];
+        const size_t n = d.n_atoms, o = d.atom_off;
+        std::vector<double> h(n);
+        const double* src[3] = {B.XB, B.YB, B.ZB};
+        for (int k = 0; k < 3; ++k) {
+            copy_sync(h.data(), src[k] + o, n * 8, hipMemcpyDeviceToHost);
+            for (size_t a = 0; a < n; ++a) build_pos[3 * a + k] = h[a];
+        }
+        SysCount c;
+        copy_sync(&c, B.d_cnt + s, sizeof(SysCount), hipMemcpyDeviceToHost);
+        *rebuild_due = c.rebuild;
     }
+    size_t bytes() const override { return footprint_bytes(B); }
     static size_t footprint_bytes(const Batch<P>& b) {
         size_t A = b.n_atoms, S = b.n_slots_cap;
         return A * (14 * 8 + 3 * 4 + 3 * sizeof(facc)) + S * (3 * 4 + sizeof(real4) + sizeof(real2) + 3 * sizeof(facc)) +

@@ -82,15 +82,7 @@ __global__ void kinetic_kernel(const int* __restrict__ atom_sys, const int* __re
         }
         ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     }
-    // wave-uniform system -> one atomic per wave
-    const int s0 = __shfl(s, 0, 64);
-    const bool uniform = __all(s == s0 || s < 0);
-    if (uniform) {
-        ke = wave_sum(ke);
-        if ((threadIdx.x & 63) == 0 && s0 >= 0) energy_add(&energy[s0 * E_NTERMS + E_KIN], ke);
-    } else if (s >= 0) {
-        energy_add(&energy[s * E_NTERMS + E_KIN], ke);
-    }
+    wave_energy_add(energy, s < 0 ? 0 : s, E_KIN, ke, s >= 0);
 }
 
 template <class P>

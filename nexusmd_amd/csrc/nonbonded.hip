@@ -277,11 +277,11 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
             }
         }
         if constexpr (ENERGY) {
-            ec_acc = wave_sum(ec_acc);
-            el_acc = wave_sum(el_acc);
+            const long long ec = wave_sum(energy_fixed(ec_acc));
+            const long long el = wave_sum(energy_fixed(el_acc));
             if (lane == 0) {
-                energy_add(&energy[s * E_NTERMS + E_COUL], ec_acc);
-                energy_add(&energy[s * E_NTERMS + E_LJ], el_acc);
+                energy_add_fixed(&energy[s * E_NTERMS + E_COUL], ec);
+                energy_add_fixed(&energy[s * E_NTERMS + E_LJ], el);
             }
         }
         wave_lds_sync();   // LDS of this wave is reused by its next chunk

@@ -105,11 +105,17 @@ __device__ __forceinline__ void facc_add(long long* p, double v) {
 __device__ __forceinline__ double facc_get(const double* p) { return *p; }
 __device__ __forceinline__ double facc_get(const long long* p) { return static_cast<double>(*p) * (1.0 / kForceScale); }
 
-// Energy accumulation: block partial (double) -> 2^30 fixed point (deterministic sum).
-__device__ __forceinline__ void energy_add(long long* e, double v) {
+// Energy accumulation in 2^30 fixed point.  Every contribution is rounded to
+// an integer on its own lane before any summation, so a system's energy is the
+// same integer whatever the grouping of its terms into waves and blocks (alone
+// or inside any batch).
+__device__ __forceinline__ long long energy_fixed(double v) {
     double s = v * kEnergyScale;
     s = fmin(fmax(s, -9.2e18), 9.2e18);
-    atomicAdd(reinterpret_cast<unsigned long long*>(e), static_cast<unsigned long long>(__double2ll_rn(s)));
+    return __double2ll_rn(s);
+}
+__device__ __forceinline__ void energy_add_fixed(long long* e, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(e), static_cast<unsigned long long>(v));
 }
 
 // ---------------------------------------------------------- wave helpers
@@ -124,19 +130,24 @@ __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+__device__ __forceinline__ long long wave_sum(long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 
-// Block-wide (<=1024 threads) sum of a double into energy slot e (one atomic per block).
-__device__ __forceinline__ void block_energy_add(long long* e, double v) {
-    __shared__ double part[16];
+// Block-wide (<=1024 threads) sum of fixed-point energies into slot e (one atomic per block).
+__device__ __forceinline__ void block_energy_add(long long* e, long long v) {
+    __shared__ long long part[16];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) part[w] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t = 0.0;
+        long long t = 0;
         const int nw = (blockDim.x + 63) >> 6;
         for (int i = 0; i < nw; ++i) t += part[i];
-        energy_add(e, t);
+        energy_add_fixed(e, t);
     }
     __syncthreads();
 }
@@ -146,13 +157,14 @@ __device__ __forceinline__ void block_energy_add(long long* e, double v) {
 __device__ __forceinline__ void wave_energy_add(long long* energy, int s, int term, double v, bool valid) {
     const unsigned long long m = __ballot(valid);
     if (m == 0ull) return;
+    const long long q = valid ? energy_fixed(v) : 0ll;
     const int lead = __ffsll((long long)m) - 1;
     const int s0 = __shfl(s, lead, 64);
     if (__all(!valid || s == s0)) {
-        v = wave_sum(valid ? v : 0.0);
-        if ((int)(threadIdx.x & 63) == lead) energy_add(&energy[s0 * E_NTERMS + term], v);
+        const long long t = wave_sum(q);
+        if ((int)(threadIdx.x & 63) == lead) energy_add_fixed(&energy[s0 * E_NTERMS + term], t);
     } else if (valid) {
-        energy_add(&energy[s * E_NTERMS + term], v);
+        energy_add_fixed(&energy[s * E_NTERMS + term], q);
     }
 }
 

@@ -31,6 +31,13 @@ struct HostSystem {
     double lb[3] = {0, 0, 0}, ub[3] = {0, 0, 0};
     std::vector<double> pos, vel;            // AoS [n][3]
     double list_scale = 1.0;                 // pair-list capacity multiplier
+    // Neighbour-list state carried across a re-layout of a batch: the positions
+    // the current list was built from (AoS; empty = build from pos) and whether
+    // a rebuild was already due.  Rebuilding from the same positions gives the
+    // same list, so a system's trajectory does not depend on when the batch
+    // around it changes.
+    std::vector<double> build_pos;
+    bool rebuild_pending = false;
 };
 
 void exclusions_from_bonds(int n, const std::vector<nx_bond>& bonds, std::vector<int>& off, std::vector<int>& idx);
@@ -54,6 +61,8 @@ class EngineBase {
     virtual const std::vector<SysCount>& counters() const = 0;
     virtual hipStream_t stream() const = 0;
     virtual void device_state(int s, double* pos, double* vel) = 0;
+    // build positions (AoS) of system s's current neighbour list and its rebuild flag
+    virtual void device_list_state(int s, double* build_pos, int* rebuild_due) = 0;
     virtual size_t bytes() const = 0;
 };
 

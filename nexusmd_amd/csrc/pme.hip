@@ -530,15 +530,15 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
     }
     __syncthreads();
     R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, tw + d.tw_off[0], -1);
-    double e = 0.0;
+    long long e = 0;   // fixed point per element: independent of the tile shape
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
         const int x = i / nt, t = i - x * nt;
         const real w = eta ? et[x * sx + t] : (real)0;
         R2 v = res[t * Kx + x];
         if constexpr (ENERGY) {
             const int kz = kz0 + t;
-            const double wt = (kz == 0 || (2 * kz == Kz)) ? 1.0 : 2.0;
-            e += wt * (double)w * ((double)v.x * v.x + (double)v.y * v.y);
+            const double wt = (kz == 0 || (2 * kz == Kz)) ? 0.5 : 1.0;
+            e += energy_fixed(wt * (double)w * ((double)v.x * v.x + (double)v.y * v.y));
         }
         v.x *= w;
         v.y *= w;
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
         const int x = i / nt, t = i - x * nt;
         c[x * sx + t] = res2[t * Kx + x];
     }
-    if constexpr (ENERGY) block_energy_add(&energy[s * E_NTERMS + E_RECIP], 0.5 * e);
+    if constexpr (ENERGY) block_energy_add(&energy[s * E_NTERMS + E_RECIP], e);
 }
 
 // Plain X pass (no convolution) for the standalone transform.

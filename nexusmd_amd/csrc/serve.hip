@@ -103,13 +103,20 @@ void sync_active_to_host(nx_server* s) {
     for (size_t i = 0; i < s->active.size(); ++i) {
         Request& r = *s->reqs[s->active[i]];
         s->eng->device_state((int)i, r.sys.pos.data(), r.sys.vel.data());
+        r.sys.build_pos.resize(r.sys.pos.size());
+        int due = 0;
+        s->eng->device_list_state((int)i, r.sys.build_pos.data(), &due);
+        r.sys.rebuild_pending = due != 0;
     }
 }
 
 void admit(nx_server* s) {
+    bool synced = false;
     while (!s->queue.empty() && (int)s->active.size() < s->max_active) {
         Request& r = *s->reqs[s->queue.front()];
         if (s->in_use + (int64_t)r.bytes > s->budget && !s->active.empty()) break;
+        // the batch is about to change: keep the running systems' device state first
+        if (!synced) { sync_active_to_host(s); synced = true; }
         s->queue.pop_front();
         r.status = 1;
         s->in_use += (int64_t)r.bytes;
@@ -257,8 +264,10 @@ nx_status nx_server_run(nx_server* s, int steps) {
             } catch (const nx::HipError& e) {
                 const int f = s->eng->failed_system();
                 if (e.code != NX_ERR_CAPACITY || f < 0 || f >= (int)s->active.size()) throw;
-                // isolate the failing request, keep the others
+                // isolate the failing request; the others completed the k steps
                 sync_active_to_host(s);
+                for (size_t i = 0; i < s->active.size(); ++i)
+                    if ((int)i != f) s->reqs[s->active[i]]->done += k;
                 Request& bad = *s->reqs[s->active[f]];
                 bad.status = 3;
                 bad.error = e.what();

@@ -14,7 +14,8 @@ Fixtures (numpy .npz, no pickled objects):
                  box rule, exclusions, sorted pair set, alpha, grid rule,
                  B-spline moduli and theta/dtheta of 16 atoms, spread grid of
                  12 atoms, per-term forces and energies, reciprocal forces at
-                 the grid rule's 64^3, and 10 leapfrog steps (dt 1 fs)
+                 the grid rule's 64^3, 10 leapfrog steps (dt 1 fs), and
+                 per-term forces and energies at the end of those steps
   water3000.npz  1,000 flexible waters (3.107 nm box, grid 32x32x30):
                  per-term forces and energies, pair count and pair-set digest
   edge.npz       tiny cases: a pair at exactly r = rc, near-collinear
@@ -100,6 +101,11 @@ def chain1500():
     out["f_recip_64"], out["e_recip_64"] = R.recip(s.pos, s.q, s.lb, s.ub, (64, 64, 64), s.cutoff, s.ewald_tol)
     pos, vel, f, rows = R.simulate(s, 10, 0.001, interval=1)
     out.update(sim_pos=pos, sim_vel=vel, sim_f=f, sim_rows=rows, sim_dt=np.float64(0.001))
+    # per-term forces at the end of the 10 steps, where no bond sits exactly at
+    # its rest length (at the generated geometry bond forces are rounding noise)
+    end = s.copy()
+    end.pos = pos
+    out.update(term_outputs(end, excl, prefix="end_"))
     return out
 
 

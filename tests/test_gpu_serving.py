@@ -1,0 +1,121 @@
+"""Continuous-batching server (nx_server) on the GPU.
+
+The property under test: a request's result does not depend on what else is
+on the GPU.  In the mixed path all force and energy sums are 64-bit fixed
+point and each system's neighbour list is rebuilt from the same positions
+whether it runs alone or in a changing batch, so alone and in the mix agree
+bit for bit.  The FP64 path sums doubles with hardware atomics in an order
+that varies from run to run, so it is held to a relative tolerance instead.
+"""
+import numpy as np
+import pytest
+
+from nexusmd_amd import _lib as L
+from nexusmd_amd import engine as E
+from nexusmd_amd import systems as S
+
+from _util import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.0005
+
+
+@pytest.fixture(scope="module")
+def requests():
+    sizes = [620, 900, 1300, 700, 1100, 800]
+    steps = [60, 140, 100, 120, 80, 90]
+    return [(S.ligand_in_water(n, seed=900 + i), k) for i, (n, k) in enumerate(zip(sizes, steps))]
+
+
+def _alone(sys_, nsteps, prec):
+    srv = E.Server(precision=prec, budget_bytes=8 << 30, max_active=4)
+    rid = srv.submit(sys_, nsteps, DT)
+    srv.run()
+    st, done = srv.poll(rid)
+    assert st == 2 and done == nsteps
+    out = srv.fetch(rid)
+    srv.close()
+    return out
+
+
+def _mix(reqs, prec, max_active=8, budget=8 << 30):
+    srv = E.Server(precision=prec, budget_bytes=budget, max_active=max_active)
+    ids = [srv.submit(s, k, DT) for s, k in reqs[:3]]
+    srv.run(50)                  # the first three are running when the rest arrive
+    for st, _ in (srv.poll(i) for i in ids):
+        assert st == 1
+    ids += [srv.submit(s, k, DT) for s, k in reqs[3:]]
+    srv.run()
+    out = []
+    for i, (s, k) in zip(ids, reqs):
+        st, done = srv.poll(i)
+        assert st == 2 and done == k
+        out.append(srv.fetch(i))
+    stats = srv.stats()
+    srv.close()
+    return out, stats
+
+
+def test_mixed_alone_equals_mix_bitwise(requests):
+    print("[gpu] serving mixed: %d requests, sizes %s" % (len(requests), [s.n for s, _ in requests]), flush=True)
+    mix, stats = _mix(requests, L.MIXED)
+    assert stats["finished"] == len(requests) and stats["failed"] == 0
+    assert stats["p99_step_ms"] >= stats["p50_step_ms"] > 0
+    for (s, k), (p, v, e) in zip(requests, mix):
+        pa, va, ea = _alone(s, k, L.MIXED)
+        assert np.all(np.isfinite(p))
+        np.testing.assert_array_equal(p, pa)
+        np.testing.assert_array_equal(v, va)
+        assert e["potential"] == ea["potential"]
+
+
+def test_fp64_alone_vs_mix_within_tolerance(requests):
+    reqs = requests[:4]
+    print("[gpu] serving fp64: sizes %s" % [s.n for s, _ in reqs], flush=True)
+    mix, _ = _mix(reqs, L.FP64)
+    for (s, k), (p, v, e) in zip(reqs, mix):
+        pa, va, ea = _alone(s, k, L.FP64)
+        assert rel_rms(p, pa) <= 1e-9
+        assert rel_rms(v, va) <= 1e-5
+        assert abs(e["potential"] - ea["potential"]) <= 1e-6 * abs(ea["potential"])
+
+
+def test_admission_by_memory_budget(requests):
+    """With a budget that holds about two requests, the rest wait and all still finish."""
+    per = []
+    probe = E.Server(precision=L.MIXED, budget_bytes=8 << 30, max_active=1)
+    for s, _ in requests:
+        probe.submit(s, 2, DT)
+        probe.run(1)                        # admitted and still running: its bytes are in use
+        per.append(probe.stats()["bytes_in_use"])
+        probe.run()
+    probe.close()
+    assert min(per) > 0
+    budget = int(2.2 * max(per))
+    srv = E.Server(precision=L.MIXED, budget_bytes=budget, max_active=64)
+    ids = [srv.submit(s, k, DT) for s, k in requests]
+    srv.run(10)
+    active = srv.stats()["active"]
+    assert 1 <= active < len(requests)
+    srv.run()
+    for i, (s, k) in zip(ids, requests):
+        assert srv.poll(i) == (2, k)
+    st = srv.stats()
+    assert st["finished"] == len(requests) and st["bytes_in_use"] == 0
+    srv.close()
+
+
+def test_bad_request_is_rejected_without_disturbing_others(requests):
+    srv = E.Server(precision=L.MIXED, budget_bytes=8 << 30, max_active=8)
+    s0, k0 = requests[0]
+    good = srv.submit(s0, k0, DT)
+    bad = s0.copy()
+    bad.bond_idx = bad.bond_idx.copy()
+    bad.bond_idx[0, 0] = bad.n + 5            # particle index out of range
+    with pytest.raises(L.NexusError) as ex:
+        srv.submit(bad, 10, DT)
+    assert ex.value.status == L.NX_ERR_ARG
+    srv.run()
+    assert srv.poll(good) == (2, k0)
+    srv.close()
