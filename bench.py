@@ -8,12 +8,14 @@ plus the leapfrog update, with the neighbour list rebuilt on the device
 whenever an atom has moved more than half the buffer.
 
 Prints ONE JSON line.  `value` is the mixed-precision ns/day over the timed
-region; the FP64 path, the force errors of both paths against the FP64 CPU
-oracle on identical positions, per-kernel times (HIP events recorded inside
-the captured step graphs), the pair-kernel roofline and a CPU baseline (the
-oracle, one core, a bounded sample of the same workload) ride along.
+region (step graphs replayed, HIP events bracketing the pair kernel of every
+step); the FP64 path, the force errors of both paths against the FP64 CPU
+oracle on identical positions, a per-kernel breakdown, the pair-kernel
+roofline, a CPU baseline (the oracle, one core, a bounded sample of the same
+workload) and the batched-serving measurement (256 ligand-in-water requests,
+continuous batching vs one at a time, p50/p99 step latency) ride along.
 
-    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--serving]
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-fp64] [--no-serving]
 """
 from __future__ import annotations
 
@@ -80,29 +82,44 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
     }
 
 
-def time_path(sysm, prec, steps, warmup, skin):
-    """Warm up, then the timed region: `steps` steps with HIP events around every
-    kernel family (plain launches).  The same count is also run as graph replays
-    without events, reported beside it."""
+def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
+    """Warm-up, then the timed region: exactly `steps` steps replayed as step
+    graphs, with HIP events on the context's stream bracketing the pair kernel
+    of every step (nx_step returns after the stream has drained, so the wall
+    clock around it covers exactly the timed steps).  Before it, the same count
+    as single-graph replays without events; after it, a shorter run with events
+    around every kernel family for the per-kernel breakdown."""
+    from nexusmd_amd import _lib as L
     from nexusmd_amd import engine as E
+    graphs = os.environ.get("NEXUSMD_GRAPHS", "1") != "0"
     ctx = E.Context(sysm, precision=prec, skin=skin)
     ctx.step(warmup, sysm.dt)
     t0 = time.perf_counter()
-    ctx.step(steps, sysm.dt)                       # hipGraph replays, no events
+    ctx.step(steps, sysm.dt)                       # one graph per step, no events
     graph_s = time.perf_counter() - t0
     pairs0 = ctx.pair_count()
-    ctx.enable_timing(True)
+    # with graphs off (NEXUSMD_GRAPHS=0) the timed region is direct launches with
+    # events around every kernel family, the pair kernel among them
+    ctx.enable_timing(L.TIMING_GRAPH_PAIR if graphs else L.TIMING_KERNELS)
+    ctx.step(2, sysm.dt)                           # builds the segment graphs outside the timed region
     t0 = time.perf_counter()
-    ctx.step(steps, sysm.dt)                       # timed region (events per kernel)
+    ctx.step(steps, sysm.dt)                       # timed region
     t1 = time.perf_counter()
+    pair_ms, pair_n = ctx.kernel_time("pair")
+    step_ms, step_n = ctx.kernel_time("step") if graphs else (1e3 * (t1 - t0) / steps, steps)
+    stats = ctx.stats()
+    pairs1 = ctx.pair_count()
+    ctx.enable_timing(L.TIMING_KERNELS)
+    nb = min(steps, breakdown_steps)
+    ctx.step(nb, sysm.dt)
     kern = {}
     for name in ("pair", "pme_spread", "fft", "pme_gather", "bonded", "exclusion", "integrate", "nlist", "pack"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"ms_per_launch": ms, "launches": n}
-    stats = ctx.stats()
-    ctx.enable_timing(False)
-    pairs1 = ctx.pair_count()
-    return ctx, (t1 - t0), kern, stats, 0.5 * (pairs0 + pairs1), graph_s
+    ctx.enable_timing(L.TIMING_OFF)
+    timed = {"pair_ms_per_launch": pair_ms, "pair_launches": pair_n, "step_ms_events": step_ms,
+             "breakdown_steps": nb, "launch": "step graphs" if graphs else "direct launches"}
+    return ctx, (t1 - t0), kern, stats, 0.5 * (pairs0 + pairs1), graph_s, timed
 
 
 def cpu_baseline(sysm, budget_s=20.0):
@@ -128,36 +145,78 @@ def cpu_baseline(sysm, budget_s=20.0):
     }
 
 
-def serving_bench(n_requests, steps_per_request, seed=20261205):
-    """Continuous batching of ligand-in-water requests vs one at a time."""
+def _make_request(args):
+    from nexusmd_amd import systems as S
+    n, sd, steps = args
+    return S.ligand_in_water(n, sd, steps=steps)
+
+
+def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, workers=16):
+    """Continuous batching of ligand-in-water requests (atom counts lognormal,
+    median 2,500, clipped to [600, 8000]) against the same requests run one at
+    a time, all in the mixed path.
+
+    Batched: requests arrive as a seeded Poisson process at 0.8x the service
+    rate measured in a short saturated calibration; the server admits them by
+    HBM footprint as they arrive and advances every active system together.
+    `aggregate_ns_per_day` = sum of simulated time / wall-clock makespan.
+    One at a time: every request runs alone (its own context, graph replays)
+    for `alone_steps` steps after warm-up; its per-step time times the
+    requested step count is its service time, and the sum of those is the
+    sequential makespan."""
+    import multiprocessing as mp
+
     from nexusmd_amd import _lib as L
     from nexusmd_amd import engine as E
     from nexusmd_amd import systems as S
     sizes, seeds = S.serving_batch(n_requests, seed=seed)
-    reqs = [S.ligand_in_water(n, sd, steps=steps_per_request) for n, sd in zip(sizes, seeds)]
+    t = time.perf_counter()
+    jobs = [(n, sd, steps_per_request) for n, sd in zip(sizes, seeds)]
+    with mp.get_context("fork").Pool(max(1, min(workers, os.cpu_count() or 1))) as pool:
+        reqs = pool.map(_make_request, jobs, chunksize=4)
+    log("serving: %d requests generated in %.1f s (%d atoms total)" % (len(reqs), time.perf_counter() - t,
+                                                                      sum(r.n for r in reqs)))
     dt = reqs[0].dt
-    # one at a time (each system alone on the GPU)
-    t0 = time.perf_counter()
-    sample = reqs[: min(8, len(reqs))]
-    for r in sample:
+    # one at a time
+    alone_step_s = []
+    for r in reqs:
         ctx = E.Context(r, precision=L.MIXED, skin=0.1)
-        ctx.step(steps_per_request, dt)
+        ctx.step(20, dt)
+        t0 = time.perf_counter()
+        ctx.step(alone_steps, dt)
+        alone_step_s.append((time.perf_counter() - t0) / alone_steps)
         ctx.close()
-    alone_s = time.perf_counter() - t0
-    alone_ns_day = ns_per_day(steps_per_request * len(sample), dt, alone_s)
-    # batched, Poisson arrivals at 0.8x the measured one-at-a-time service rate
-    rate = 0.8 * len(sample) / alone_s * 4.0
+    alone_makespan = steps_per_request * float(np.sum(alone_step_s))
+    sim_ns = len(reqs) * steps_per_request * dt * 1e-3
+    alone_ns_day = 86400.0 * sim_ns / alone_makespan
+    log("serving: one at a time %.1f ns/day aggregate (median step %.3f ms)" % (
+        alone_ns_day, 1e3 * float(np.median(alone_step_s))))
+    # service-rate calibration: the first 64 requests together, saturated
+    cal = E.Server(precision=L.MIXED, budget_bytes=200 << 30, max_active=n_requests)
+    cal_reqs = reqs[: min(64, len(reqs))]
+    for r in cal_reqs:
+        cal.submit(r, 200, dt)
+    cal.run(20)
+    t0 = time.perf_counter()
+    cal.run(100)
+    cal_s = time.perf_counter() - t0
+    cal.close()
+    atom_steps_per_s = 100 * sum(r.n for r in cal_reqs) / cal_s
+    service_rate = atom_steps_per_s / (float(np.mean([r.n for r in reqs])) * steps_per_request)
+    rate = 0.8 * service_rate
     arrivals = S.poisson_arrivals(len(reqs), rate, seed + 1)
-    srv = E.Server(precision=L.MIXED, budget_bytes=200 << 30, max_active=256)
+    # batched
+    srv = E.Server(precision=L.MIXED, budget_bytes=200 << 30, max_active=n_requests)
     t0 = time.perf_counter()
     nxt = 0
-    ids = []
+    max_active = 0
     while True:
         now = time.perf_counter() - t0
         while nxt < len(reqs) and arrivals[nxt] <= now:
-            ids.append(srv.submit(reqs[nxt], steps_per_request, dt))
+            srv.submit(reqs[nxt], steps_per_request, dt)
             nxt += 1
         st = srv.stats()
+        max_active = max(max_active, st["active"])
         if nxt >= len(reqs) and st["finished"] + st["failed"] >= len(reqs):
             break
         if st["active"] == 0 and st["submitted"] == st["finished"] + st["failed"]:
@@ -167,20 +226,45 @@ def serving_bench(n_requests, steps_per_request, seed=20261205):
         srv.run(50)
     wall = time.perf_counter() - t0
     st = srv.stats()
-    total_ns = steps_per_request * dt * 1e-3 * st["finished"]
+    srv.close()
+    done_ns = steps_per_request * dt * 1e-3 * st["finished"]
+    agg = 86400.0 * done_ns / wall
     return {
         "requests": len(reqs),
         "atoms_min_median_max": [int(min(sizes)), int(np.median(sizes)), int(max(sizes))],
+        "atoms_total": int(sum(r.n for r in reqs)),
         "steps_per_request": steps_per_request,
-        "aggregate_ns_per_day": 86400.0 * total_ns / wall,
-        "device_aggregate_ns_per_day": st["aggregate_ns_per_day"],
+        "dt_fs": dt * 1e3,
+        "arrival_rate_per_s": rate,
+        "max_active": int(max_active),
+        "aggregate_ns_per_day": agg,
         "one_at_a_time_ns_per_day": alone_ns_day,
-        "batched_over_alone": (86400.0 * total_ns / wall) / alone_ns_day,
+        "batched_over_one_at_a_time": agg / alone_ns_day,
         "p50_step_ms": st["p50_step_ms"],
         "p99_step_ms": st["p99_step_ms"],
+        "finished": st["finished"],
         "failed": st["failed"],
-        "wall_s": wall,
+        "makespan_s": wall,
+        "server_step_s": st["wall_s"],
+        "one_at_a_time_makespan_s": alone_makespan,
+        "one_at_a_time_sample": "every request alone for %d steps after 20 warm-up steps; its per-step time "
+                                "x %d steps is its service time" % (alone_steps, steps_per_request),
     }
+
+
+def copy_calibration(mib=256, reps=3):
+    """A plain copy kernel of known size: 256 MiB read and 256 MiB written per
+    launch, to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: %d launches of a %d MiB device copy" % (reps, mib))
+    del a, b
+    torch.cuda.empty_cache()
 
 
 def main():
@@ -193,9 +277,15 @@ def main():
     ap.add_argument("--relax", type=int, default=300, help="steepest-descent iterations before timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--serving", action="store_true", help="also run the batched-serving measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving measurement")
+    ap.add_argument("--serving-only", action="store_true", help="run only the batched-serving measurement")
     ap.add_argument("--serving-requests", type=int, default=256)
-    ap.add_argument("--serving-steps", type=int, default=500)
+    ap.add_argument("--serving-steps", type=int, default=5000)
+    ap.add_argument("--alone-steps", type=int, default=200)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="direct kernel launches instead of step graphs (sets NEXUSMD_GRAPHS=0)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every run here uses a single "
@@ -204,6 +294,15 @@ def main():
 
     from nexusmd_amd import _lib as L
     from nexusmd_amd import engine as E
+
+    if args.no_graphs:
+        os.environ["NEXUSMD_GRAPHS"] = "0"
+    if args.calibrate_copy:
+        copy_calibration()
+    if args.serving_only:
+        sv = serving_bench(args.serving_requests, args.serving_steps, args.alone_steps)
+        print(json.dumps({"serving": sv}), flush=True)
+        return
 
     t = time.perf_counter()
     sysm = make_workload(args.workload)
@@ -229,22 +328,25 @@ def main():
         c.close()
     log("force errors", errs)
 
-    ctx_m, sec_m, kern_m, stats_m, pairs, graph_m = time_path(sysm, L.MIXED, args.steps, args.warmup, args.skin)
+    ctx_m, sec_m, kern_m, stats_m, pairs, graph_m, timed_m = time_path(sysm, L.MIXED, args.steps, args.warmup,
+                                                                       args.skin)
     ctx_m.close()
     nsd_m = ns_per_day(args.steps, sysm.dt, sec_m)
     log("mixed: %.3f ms/step, %.2f ns/day" % (1e3 * sec_m / args.steps, nsd_m))
     fp64 = None
     if not args.no_fp64:
         steps64 = max(10, args.steps // 4)
-        ctx_d, sec_d, kern_d, stats_d, _, graph_d = time_path(sysm, L.FP64, steps64, max(5, args.warmup // 4), args.skin)
+        ctx_d, sec_d, kern_d, stats_d, _, graph_d, timed_d = time_path(sysm, L.FP64, steps64,
+                                                                       max(5, args.warmup // 4), args.skin)
         ctx_d.close()
         nsd_d = ns_per_day(steps64, sysm.dt, sec_d)
         fp64 = {"ns_per_day": nsd_d, "ms_per_step": 1e3 * sec_d / steps64, "steps": steps64,
-                "graph_replay_ns_per_day": ns_per_day(steps64, sysm.dt, graph_d), "kernels": kern_d}
+                "graph_replay_ns_per_day": ns_per_day(steps64, sysm.dt, graph_d),
+                "pair_ms_per_launch": timed_d["pair_ms_per_launch"], "kernels": kern_d}
         log("fp64: %.3f ms/step, %.2f ns/day" % (1e3 * sec_d / steps64, nsd_d))
 
     # pair-kernel roofline (FP32 VALU; mixed path)
-    pair_ms = kern_m["pair"]["ms_per_launch"]
+    pair_ms = timed_m["pair_ms_per_launch"]
     flops = PAIR_FLOP * pairs
     achieved = flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else 0.0
     tiles = stats_m["tiles"]
@@ -296,14 +398,15 @@ def main():
         "force_error_vs_fp64_oracle": errs,
         "roofline": roofline,
         "kernels_mixed": kernels,
+        "timed_region": timed_m,
         "pair_list": {"entries": stats_m["list_entries"], "tiles": tiles, "rebuilds": stats_m["rebuilds"]},
     }
     if not args.no_cpu:
         cb = cpu_baseline(sysm)
         out["cpu_baseline"] = cb
         log("cpu baseline %.4f ns/day" % cb["value"])
-    if args.serving:
-        out["serving"] = serving_bench(args.serving_requests, args.serving_steps)
+    if not args.no_serving:
+        out["serving"] = serving_bench(args.serving_requests, args.serving_steps, args.alone_steps)
         log("serving", out["serving"])
     print(json.dumps(out), flush=True)
 

@@ -131,11 +131,21 @@ nx_status nx_step_report(nx_ctx* ctx, double dt, int force_mask, double* pos, do
  * Writes up to cap pairs; *count receives the full count. */
 nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count);
 
-/* Kernel timing of the last nx_step/nx_compute_forces call, per kernel
- * family, in milliseconds per launch (average over launches in the call).
- * names: "pair", "pme_spread", "fft", "pme_gather", "bonded", "integrate",
- * "nlist", "exclusion".  Timing is only collected when enabled. */
-nx_status nx_enable_timing(nx_ctx* ctx, int on);
+/* Kernel timing of the last nx_step/nx_compute_forces call, in milliseconds
+ * per launch (average over launches in the call), from HIP events on the
+ * context's stream.  Modes:
+ *   NX_TIMING_KERNELS     every kernel family launched directly with events
+ *                         around it: "nlist", "pack", "bonded", "exclusion",
+ *                         "pair", "pme_spread", "fft", "pme_gather", "integrate"
+ *   NX_TIMING_GRAPH_PAIR  nx_step replays each step as three graphs (before
+ *                         the pair kernel, the pair kernel, after it) with
+ *                         events between them: "pair" and "step"
+ * Setting NEXUSMD_GRAPHS=0 in the environment before nx_create makes every
+ * step run as direct launches (NX_TIMING_GRAPH_PAIR then times as
+ * NX_TIMING_KERNELS).
+ * No counterpart in the reference (it has no GPU path to time). */
+enum { NX_TIMING_OFF = 0, NX_TIMING_KERNELS = 1, NX_TIMING_GRAPH_PAIR = 2 };
+nx_status nx_enable_timing(nx_ctx* ctx, int mode);
 nx_status nx_kernel_time(nx_ctx* ctx, const char* name, double* ms_per_launch, int64_t* launches);
 
 /* Counters of the last call: pairs evaluated by the pair kernel (cluster

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnexusmd_hip.so")
 
 NX_OK, NX_ERR_ARG, NX_ERR_HIP, NX_ERR_CAPACITY, NX_ERR_STATE, NX_ERR_NOMEM, NX_ERR_NUMERIC = range(7)
 FP64, MIXED = 0, 1
+TIMING_OFF, TIMING_KERNELS, TIMING_GRAPH_PAIR = 0, 1, 2
 BOND, ANGLE, TORSION, NONBONDED, ALL = 1, 2, 4, 8, 15
 
 D = C.POINTER(C.c_double)

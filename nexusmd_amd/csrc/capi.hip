@@ -18,7 +18,7 @@ struct nx_ctx {
     std::unique_ptr<nx::EngineBase> eng;
     nx::HostSystem sys;
     bool have_top = false, have_box = false, have_state = false, dirty = true;
-    bool timing = false;
+    int timing = NX_TIMING_OFF;
     std::string err;
 };
 
@@ -299,7 +299,8 @@ nx_status nx_neighbor_pairs(nx_ctx* c, int* pairs, int64_t cap, int64_t* count) 
 
 nx_status nx_enable_timing(nx_ctx* c, int on) {
     return guarded(c, [&] {
-        c->timing = on != 0;
+        if (on < NX_TIMING_OFF || on > NX_TIMING_GRAPH_PAIR) throw ArgError("unknown timing mode");
+        c->timing = on;
         if (c->eng) c->eng->set_timing(c->timing);
     });
 }

@@ -16,6 +16,8 @@
 #include <vector>
 
 #include "../../include/nexusmd.h"
+#include <cstdlib>
+
 #include "nx_device.h"
 #include "nx_engine.h"
 #include "nx_host.h"
@@ -124,7 +126,7 @@ static bool fft_size_ok(int K) {
     return m == 1;
 }
 
-// PME grid rule Q2 (NonbondedForce.cpp:33-48).
+// PME grid rule: next power of two >= 10 * ceil(L) per axis (NonbondedForce.cpp:33-48).
 static int grid_rule(double L) {
     int m = (int)ceil(L) * 10, p = 1;
     while (p < m) p *= 2;
@@ -144,6 +146,9 @@ class Engine : public EngineBase {
     Engine(int device) : device_(device) {
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+        // NEXUSMD_GRAPHS=0: every step as direct launches (no graph capture or replay)
+        const char* g = std::getenv("NEXUSMD_GRAPHS");
+        if (g && std::string(g) == "0") graphs_enabled_ = false;
     }
     ~Engine() override {
         hipSetDevice(device_);
@@ -155,6 +160,7 @@ class Engine : public EngineBase {
     void release() {
         if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
         if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+        destroy_segments();
         for (void* p : owned_) hipFree(p);
         owned_.clear();
         Batch<P> fresh;
@@ -606,13 +612,18 @@ class Engine : public EngineBase {
         ev_used_ = 0;
     }
 
-    void forces(int mask, bool energy) {
-        timed("nlist", [&] { launch_nlist<P>(B); });
-        timed("pack", [&] { launch_pack<P>(B); });
-        if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
-        if (mask & NX_FORCE_NONBONDED) {
-            timed("exclusion", [&] { launch_exclusion<P>(B, energy); });
-            timed("pair", [&] { launch_pair<P>(B, energy); });
+    // One force evaluation in three parts: 0 = list, packing, bonded and
+    // exclusion terms; 1 = the pair kernel; 2 = PME.  -1 = all of them.
+    void forces_part(int mask, bool energy, int part) {
+        if (part <= 0) {
+            timed("nlist", [&] { launch_nlist<P>(B); });
+            timed("pack", [&] { launch_pack<P>(B); });
+            if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
+            if (mask & NX_FORCE_NONBONDED) timed("exclusion", [&] { launch_exclusion<P>(B, energy); });
+        }
+        if (!(mask & NX_FORCE_NONBONDED)) return;
+        if (part < 0 || part == 1) timed("pair", [&] { launch_pair<P>(B, energy); });
+        if (part < 0 || part == 2) {
             timed("pme_spread", [&] { launch_pme_spread<P>(B); });
             timed("fft", [&] {
                 launch_fft_forward<P>(B);
@@ -622,6 +633,7 @@ class Engine : public EngineBase {
             timed("pme_gather", [&] { launch_pme_gather<P>(B); });
         }
     }
+    void forces(int mask, bool energy) { forces_part(mask, energy, -1); }
 
     void zero_energy() { HIPCHK(hipMemsetAsync(B.energy, 0, sizeof(long long) * B.nsys * E_NTERMS, B.stream)); }
 
@@ -660,7 +672,26 @@ class Engine : public EngineBase {
 
     void run(int nsteps, double dt, int mask) override {
         begin_call();
-        if (B.timing) {
+        if (B.timing == NX_TIMING_GRAPH_PAIR && nsteps > 0 && graphs_enabled_) {
+            // the step as three graphs (before the pair kernel, the pair kernel,
+            // after it) with events on the stream between the replays
+            if (!seg_exec_[0] || graph_dt_seg_ != dt || graph_mask_seg_ != mask) build_segments(dt, mask);
+            hipEvent_t e0 = ev();
+            HIPCHK(hipEventRecord(e0, B.stream));
+            for (int i = 0; i < nsteps; ++i) {
+                hipEvent_t a = ev(), b = ev(), c = ev();
+                HIPCHK(hipGraphLaunch(seg_exec_[0], B.stream));
+                HIPCHK(hipEventRecord(a, B.stream));
+                HIPCHK(hipGraphLaunch(seg_exec_[1], B.stream));
+                HIPCHK(hipEventRecord(b, B.stream));
+                HIPCHK(hipGraphLaunch(seg_exec_[2], B.stream));
+                HIPCHK(hipEventRecord(c, B.stream));
+                evs_.push_back({"pair", a, b});
+                evs_.push_back({"step", e0, c});
+                e0 = c;
+            }
+            steps_ += nsteps;
+        } else if (B.timing) {
             // per-kernel HIP events need plain launches (events recorded in a graph
             // cannot be queried for elapsed time); the host enqueues ahead of the GPU
             for (int i = 0; i < nsteps; ++i) step_plain(dt, mask);
@@ -687,6 +718,32 @@ class Engine : public EngineBase {
         HIPCHK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
         graph_dt_ = dt;
         graph_mask_ = mask;
+    }
+
+    void build_segments(double dt, int mask) {
+        destroy_segments();
+        const int saved = B.timing;
+        B.timing = 0;
+        capturing_ = true;
+        for (int g = 0; g < 3; ++g) {
+            HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+            if (g == 0) forces_part(mask, false, 0);
+            if (g == 1) forces_part(mask, false, 1);
+            if (g == 2) { forces_part(mask, false, 2); launch_integrate<P>(B, dt); }
+            hipError_t e = hipStreamEndCapture(B.stream, &seg_[g]);
+            if (e != hipSuccess) { capturing_ = false; B.timing = saved; HIPCHK(e); }
+            HIPCHK(hipGraphInstantiate(&seg_exec_[g], seg_[g], nullptr, nullptr, 0));
+        }
+        capturing_ = false;
+        B.timing = saved;
+        graph_dt_seg_ = dt;
+        graph_mask_seg_ = mask;
+    }
+    void destroy_segments() {
+        for (int g = 0; g < 3; ++g) {
+            if (seg_exec_[g]) { hipGraphExecDestroy(seg_exec_[g]); seg_exec_[g] = nullptr; }
+            if (seg_[g]) { hipGraphDestroy(seg_[g]); seg_[g] = nullptr; }
+        }
     }
 
     void energies_out(int s, int mask, nx_energies* e) {
@@ -852,7 +909,7 @@ class Engine : public EngineBase {
         return true;
     }
 
-    void set_timing(bool on) override { B.timing = on; }
+    void set_timing(int mode) override { B.timing = mode; }
     void set_graphs(bool on) override { graphs_enabled_ = on; }
     int nsys() const override { return B.nsys; }
     int failed_system() const override { return failed_sys_; }
@@ -913,6 +970,10 @@ class Engine : public EngineBase {
     int graph_mask_ = -1;
     int failed_sys_ = -1;
     int grow_depth_ = 0;
+    hipGraph_t seg_[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t seg_exec_[3] = {nullptr, nullptr, nullptr};
+    double graph_dt_seg_ = 0.0;
+    int graph_mask_seg_ = -1;
     std::vector<SysCount> counts_;
 };
 

@@ -116,6 +116,16 @@ __global__ void pair_prefix_kernel(const SysDev* __restrict__ sd, const SysCount
 
 constexpr int kPairWaves = 4;   // independent wavefronts per 256-thread block
 
+// Lanes l = 8 * ia + ja with ja <= ia: on a diagonal tile (i and j the same
+// cluster) these are the self pairs and the half counted from the other side.
+constexpr unsigned long long lower_triangle_mask() {
+    unsigned long long m = 0ull;
+    for (int l = 0; l < 64; ++l)
+        if ((l & 7) <= (l >> 3)) m |= 1ull << l;
+    return m;
+}
+constexpr unsigned long long kLowerTriangle = lower_triangle_mask();
+
 template <class P, bool ENERGY>
 __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd, const int* __restrict__ prefix,
                                                    int nsys, const int4* __restrict__ chunks,
@@ -145,33 +155,62 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
     const int g0 = vw * per, g1 = min(total, g0 + per);
     const real ke = (real)kOne4PiEps0;
 
-    for (int g = g0; g < g1; ++g) {
+    // Everything a chunk needs from memory before its first pair: the chunk
+    // record, the i data of its clusters and its first list entry.  The next
+    // chunk's copy is loaded before this chunk's force atomics are issued, so
+    // the wait for it does not sit behind them (one in-order vmem counter).
+    struct Pre {
+        int s;
+        int4 ch;
+        real4 p;
+        real2 l;
+        uint2 ent;
+        real4 pj;
+        real2 ljj;
+    };
+    auto fetch = [&](int g) {
+        Pre q;
         int lo = 0, hi = nsys - 1;     // largest s with prefix[s] <= g
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
         }
-        const int s = lo;
-        const int4 ch = chunks[sd[s].chunk_off + (g - prefix[s])];
-        const int fc = ch.x, ncl = ch.y, e0 = ch.z, e1 = ch.w;
+        q.s = lo;
+        q.ch = chunks[sd[lo].chunk_off + (g - prefix[lo])];
+        if ((lane >> 3) < q.ch.y) {
+            q.p = pos[q.ch.x * 8 + lane];
+            q.l = lj[q.ch.x * 8 + lane];
+        } else {
+            q.p.x = q.p.y = q.p.z = q.p.w = 0;
+            q.l.x = q.l.y = 0;
+        }
+        q.ent = list[q.ch.z];
+        const int sj = (int)(q.ent.x & 0xffffffu) * 8 + ja;
+        q.pj = pos[sj];
+        q.ljj = lj[sj];
+        return q;
+    };
+
+    // The mixed path prefetches the next chunk across the flush below; the FP64
+    // kernel is register-bound and fetches at the top of each chunk.
+    Pre cur;
+    if (P::mixed && g0 < g1) cur = fetch(g0);
+    for (int g = g0; g < g1; ++g) {
+        if constexpr (!P::mixed) cur = fetch(g);
+        // chunk values are the same on every lane: keep them in scalar registers
+        const int s = __builtin_amdgcn_readfirstlane(cur.s);
+        const int fc = __builtin_amdgcn_readfirstlane(cur.ch.x), ncl = __builtin_amdgcn_readfirstlane(cur.ch.y);
+        const int e0 = __builtin_amdgcn_readfirstlane(cur.ch.z), e1 = __builtin_amdgcn_readfirstlane(cur.ch.w);
         const real Lx = (real)sd[s].L[0], Ly = (real)sd[s].L[1], Lz = (real)sd[s].L[2];
+        const real iLx = (real)sd[s].invL[0], iLy = (real)sd[s].invL[1], iLz = (real)sd[s].invL[2];
         const real rc2 = (real)sd[s].rc2;
         const real alpha = (real)sd[s].alpha;
         const real sw_on = (real)sd[s].sw_on, sw_inv = (real)sd[s].sw_inv;
         {
-            const int k = lane >> 3;
-            real4 p;
-            real2 l;
-            if (k < ncl) {
-                p = pos[fc * 8 + lane];
-                l = lj[fc * 8 + lane];
-                p.w *= ke;
-            } else {
-                p.x = p.y = p.z = p.w = 0;
-                l.x = l.y = 0;
-            }
+            real4 p = cur.p;
+            p.w *= ke;
             ipos[lane] = p;
-            ilj[lane] = l;
+            ilj[lane] = cur.l;
         }
         wave_lds_sync();
         real fxi[8], fyi[8], fzi[8];
@@ -180,13 +219,24 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
         double ec_acc = 0.0, el_acc = 0.0;
 
         // software pipeline: entry e+1 (list word and j data) loads while entry e computes
-        uint2 ent = list[e0];
-        real4 pj = pos[(int)(ent.x & 0xffffffu) * 8 + ja];
-        real2 lj_j = lj[(int)(ent.x & 0xffffffu) * 8 + ja];
+        uint2 ent = cur.ent;
+        real4 pj = cur.pj;
+        real2 lj_j = cur.ljj;
         for (int e = e0; e < e1; ++e) {
-            const int cj = (int)(ent.x & 0xffffffu);
-            const unsigned im = ent.x >> 24;
-            const unsigned eid = ent.y;
+            // every lane loaded the same list word: move it to scalar registers so the
+            // per-cluster tests below are scalar branches
+            const unsigned ex = __builtin_amdgcn_readfirstlane(ent.x);
+            const unsigned eid = __builtin_amdgcn_readfirstlane(ent.y);
+            const int cj = (int)(ex & 0xffffffu);
+            const unsigned im = ex >> 24;
+            // exclusion bits of the 8 cluster pairs, one load before the cluster loop
+            unsigned long long em[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) em[k] = 0ull;
+            if (eid != 0xffffffffu) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) em[k] = emask[(size_t)eid * 8 + k];
+            }
             uint2 ent_n = ent;
             real4 pj_n = pj;
             real2 lj_n = lj_j;
@@ -200,13 +250,23 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if (im & (1u << k)) {
-                    bool excl = (cj == fc + k) && (ja <= ia);
-                    if (eid != 0xffffffffu) excl |= ((emask[(size_t)eid * 8 + k] >> lane) & 1ull) != 0ull;
+                    // excluded lanes: listed exclusions, and on the diagonal tile the
+                    // pairs with ja <= ia (self pairs and the other half)
+                    const bool excl = (((em[k] | (cj == fc + k ? kLowerTriangle : 0ull)) >> lane) & 1ull) != 0ull;
                     const real4 pi = ipos[k * 8 + ia];
                     const real2 li = ilj[k * 8 + ia];
-                    const real dx = min_image1(pi.x - pj.x, Lx);
-                    const real dy = min_image1(pi.y - pj.y, Ly);
-                    const real dz = min_image1(pi.z - pj.z, Lz);
+                    real dx, dy, dz;
+                    if constexpr (P::mixed) {
+                        // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
+                        const real ux = pi.x - pj.x, uy = pi.y - pj.y, uz = pi.z - pj.z;
+                        dx = ux - Lx * rintf(ux * iLx);
+                        dy = uy - Ly * rintf(uy * iLy);
+                        dz = uz - Lz * rintf(uz * iLz);
+                    } else {
+                        dx = min_image1(pi.x - pj.x, Lx);
+                        dy = min_image1(pi.y - pj.y, Ly);
+                        dz = min_image1(pi.z - pj.z, Lz);
+                    }
                     real r2;
                     if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
                     else r2 = dot_rn(dx, dy, dz);
@@ -238,7 +298,7 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
             fyj = sum_stride8(fyj);
             fzj = sum_stride8(fzj);
             // staged in LDS: no global atomics inside the loop, so waiting for the
-            // prefetched loads never waits behind atomics (one in-order vmem counter)
+            // prefetched loads never waits behind atomics
             const int el = e - e0;
             if (ia == 0) {
                 jf[(el * 3 + 0) * 8 + ja] = fxj;
@@ -249,6 +309,9 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
             ent = ent_n;
             pj = pj_n;
             lj_j = lj_n;
+        }
+        if constexpr (P::mixed) {
+            if (g + 1 < g1) cur = fetch(g + 1);
         }
         wave_lds_sync();
         {
@@ -264,17 +327,24 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
                 }
             }
         }
+        // i-forces: sums over the 8 j lanes, transposed through LDS so that one
+        // full-wave atomic per component covers the chunk's 64 i atoms
+        wave_lds_sync();   // the j flush has read jf
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            if (k < ncl) {
-                const real fx = sum_group8(fxi[k]), fy = sum_group8(fyi[k]), fz = sum_group8(fzi[k]);
-                if (ja == 0) {
-                    const int slot = (fc + k) * 8 + ia;
-                    facc_add(&FX[slot], fx);
-                    facc_add(&FY[slot], fy);
-                    facc_add(&FZ[slot], fz);
-                }
+            const real fx = sum_group8(fxi[k]), fy = sum_group8(fyi[k]), fz = sum_group8(fzi[k]);
+            if (ja == 0) {
+                jf[0 * 64 + k * 8 + ia] = fx;
+                jf[1 * 64 + k * 8 + ia] = fy;
+                jf[2 * 64 + k * 8 + ia] = fz;
             }
+        }
+        wave_lds_sync();
+        if ((lane >> 3) < ncl) {
+            const int slot = fc * 8 + lane;
+            facc_add(&FX[slot], jf[0 * 64 + lane]);
+            facc_add(&FY[slot], jf[1 * 64 + lane]);
+            facc_add(&FZ[slot], jf[2 * 64 + lane]);
         }
         if constexpr (ENERGY) {
             const long long ec = wave_sum(energy_fixed(ec_acc));

@@ -94,7 +94,7 @@ struct Batch {
     long long* energy = nullptr;
 
     // flags
-    bool timing = false;
+    int timing = 0;                 // NX_TIMING_* mode
 };
 
 template <class P> void launch_nlist(Batch<P>& B);

@@ -54,7 +54,7 @@ class EngineBase {
     virtual long long export_pairs(std::vector<int>* out) = 0;   // nullptr: count only
     virtual void stats(nx_stats* st) = 0;
     virtual bool kernel_time(const std::string& name, double* ms, long long* n) = 0;
-    virtual void set_timing(bool on) = 0;
+    virtual void set_timing(int mode) = 0;   // NX_TIMING_*
     virtual void set_graphs(bool on) = 0;
     virtual int nsys() const = 0;
     virtual int failed_system() const = 0;

@@ -147,8 +147,11 @@ class Context:
         self._check(self.lib.nx_neighbor_pairs(self.h, None, 0, C.byref(cnt)), "nx_neighbor_pairs")
         return int(cnt.value)
 
-    def enable_timing(self, on=True):
-        self._check(self.lib.nx_enable_timing(self.h, 1 if on else 0), "nx_enable_timing")
+    def enable_timing(self, mode=L.TIMING_KERNELS):
+        """mode: L.TIMING_OFF, L.TIMING_KERNELS (every kernel family, direct launches) or
+        L.TIMING_GRAPH_PAIR (graph-replayed steps with the pair kernel bracketed)."""
+        mode = int(mode) if not isinstance(mode, bool) else (L.TIMING_KERNELS if mode else L.TIMING_OFF)
+        self._check(self.lib.nx_enable_timing(self.h, mode), "nx_enable_timing")
 
     def kernel_time(self, name):
         ms = C.c_double(0)

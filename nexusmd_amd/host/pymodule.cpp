@@ -1,0 +1,90 @@
+// pymodule.cpp -- the Python module `NexusMD`, exposing the same names as the
+// reference's SWIG wrapper (wrapper/NexusMD.i: TaskDispatcher with its factory,
+// the Enable*Force calls and Simulate; Coords3D; the SystemXMLParser statics),
+// so that scripts written for the reference (test/NexusMDTest.py) run
+// unchanged.  C++ exceptions surface as Python RuntimeError / ValueError.
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "nexusmd_host.h"
+
+namespace py = pybind11;
+using namespace nexusmd;
+
+PYBIND11_MODULE(NexusMD, m) {
+    m.doc() = "NexusMD force-and-integrate engine on AMD Instinct MI355X (HIP, gfx950)";
+
+    py::class_<TaskDispatcher, std::shared_ptr<TaskDispatcher>>(m, "TaskDispatcher")
+        .def_static("CreateDispatcher", &TaskDispatcher::CreateDispatcher, py::arg("processorChoice"))
+        .def("EnableHarmonicBondForce", &TaskDispatcher::EnableHarmonicBondForce)
+        .def("EnableHarmonicAngleForce", &TaskDispatcher::EnableHarmonicAngleForce)
+        .def("EnablePeriodicTorsionForce", &TaskDispatcher::EnablePeriodicTorsionForce)
+        .def("EnableNonbondedForce", &TaskDispatcher::EnableNonbondedForce)
+        .def("Simulate", &TaskDispatcher::Simulate, py::arg("systemFilename"), py::arg("stateFilename"),
+             py::arg("inputFilename"), py::arg("outputFilename"), py::arg("StepSize"), py::arg("TotalSteps"),
+             py::arg("interval"), py::call_guard<py::gil_scoped_release>())
+        .def_readwrite("device", &TaskDispatcher::device)
+        .def_readwrite("skin", &TaskDispatcher::skin)
+        .def_readwrite("echo_steps", &TaskDispatcher::echo_steps);
+
+    py::class_<BondParams>(m, "BondParams")
+        .def_readonly("p1", &BondParams::p1).def_readonly("p2", &BondParams::p2)
+        .def_readonly("d", &BondParams::d).def_readonly("k", &BondParams::k);
+    py::class_<AngleParams>(m, "AngleParams")
+        .def_readonly("p1", &AngleParams::p1).def_readonly("p2", &AngleParams::p2)
+        .def_readonly("p3", &AngleParams::p3).def_readonly("a", &AngleParams::a)
+        .def_readonly("k", &AngleParams::k);
+    py::class_<PTorsionParams>(m, "PTorsionParams")
+        .def_readonly("p1", &PTorsionParams::p1).def_readonly("p2", &PTorsionParams::p2)
+        .def_readonly("p3", &PTorsionParams::p3).def_readonly("p4", &PTorsionParams::p4)
+        .def_readonly("k", &PTorsionParams::k).def_readonly("phase", &PTorsionParams::phase)
+        .def_readonly("periodicity", &PTorsionParams::periodicity);
+    py::class_<NonbondedParticle>(m, "NonbondedParticle")
+        .def_readonly("q", &NonbondedParticle::q).def_readonly("sig", &NonbondedParticle::sig)
+        .def_readonly("eps", &NonbondedParticle::eps);
+    py::class_<NonbondedException>(m, "NonbondedException")
+        .def_readonly("p1", &NonbondedException::p1).def_readonly("p2", &NonbondedException::p2)
+        .def_readonly("q", &NonbondedException::q).def_readonly("sig", &NonbondedException::sig)
+        .def_readonly("eps", &NonbondedException::eps);
+    py::class_<NonbondedParams>(m, "NonbondedParams")
+        .def_readonly("alpha", &NonbondedParams::alpha)
+        .def_readonly("cutoff", &NonbondedParams::cutoff)
+        .def_readonly("ewaldTolerance", &NonbondedParams::ewaldTolerance)
+        .def_readonly("rfDielectric", &NonbondedParams::rfDielectric)
+        .def_readonly("switchingDistance", &NonbondedParams::switchingDistance)
+        .def_readonly("useSwitchingFunction", &NonbondedParams::useSwitchingFunction)
+        .def_readonly("method", &NonbondedParams::method)
+        .def_readonly("nx", &NonbondedParams::nx).def_readonly("ny", &NonbondedParams::ny)
+        .def_readonly("nz", &NonbondedParams::nz)
+        .def_readonly("particles", &NonbondedParams::particles)
+        .def_readonly("exceptions", &NonbondedParams::exceptions);
+
+    py::class_<SystemXMLParser>(m, "SystemXMLParser")
+        .def_static("MassesParser", &SystemXMLParser::MassesParser)
+        .def_static("PTorsionParser", &SystemXMLParser::PTorsionParser)
+        .def_static("HBondParser", &SystemXMLParser::HBondParser)
+        .def_static("HAngleParser", &SystemXMLParser::HAngleParser)
+        .def_static("NonBondedParser", &SystemXMLParser::NonBondedParser);
+    py::class_<StateXMLParser>(m, "StateXMLParser").def_static("parseStateXML", &StateXMLParser::parseStateXML);
+
+    py::class_<Reporter>(m, "Reporter")
+        .def(py::init<>())
+        .def_readwrite("date_override", &Reporter::date_override)
+        .def("TestPVFReport", &Reporter::TestPVFReport)
+        .def("TotalEnergyReport", &Reporter::TotalEnergyReport)
+        .def("pdbOutputGeneratorPart1",
+             [](Reporter& r, const std::string& in, const std::string& out) {
+                 std::vector<PDBAtom> t;
+                 r.pdbOutputGeneratorPart1(in, out, t);
+                 return t.size();
+             })
+        .def("pdbReport",
+             [](Reporter& r, const std::string& in, const std::string& out, const std::vector<Coords3D>& pos,
+                int model, bool first) {
+                 // Part1 (when first), then Part2 with the template of `in`
+                 std::vector<PDBAtom> t;
+                 if (first) r.pdbOutputGeneratorPart1(in, out, t);
+                 else r.outputTemplateExtractor(in, t);
+                 r.pdbOutputGeneratorPart2(out, t, pos, model);
+             });
+}

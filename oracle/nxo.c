@@ -358,7 +358,7 @@ double nxo_torsions(int nt, const nxo_torsion* t, const double* pos, const nxo_b
 /* NonbondedForce.cpp:293-321, "approximation" branch (:297-300). */
 double nxo_alpha(double cutoff, double tol) { return (1.0 / cutoff) * sqrt(-log(2.0 * tol)); }
 
-/* NonbondedForce.cpp:33-48 (rule Q2): next power of two >= 10*ceil(L). */
+/* NonbondedForce.cpp:33-48: next power of two >= 10*ceil(L). */
 void nxo_grid_rule(const nxo_box* box, int grid[3]) {
     for (int d = 0; d < 3; ++d) {
         int m = (int)ceil(box->L[d]) * 10;
@@ -389,7 +389,7 @@ static void bspline_knots(int order, double* data) {
 }
 
 /* |b(m)|^2 moduli by an O(K^2) DFT with the neighbour-average fix-up
- * (NonbondedForce.cpp:112-142, quirk Q12). */
+ * (NonbondedForce.cpp:112-142; for order 5 on even K this fires at K/2). */
 void nxo_bspline_moduli(int K, int order, double* mod) {
     double data[16];
     bspline_knots(order, data);
@@ -712,7 +712,7 @@ int nxo_compute_forces(const nxo_system* s, const double* pos, int mask, double*
         e->exclusion = nxo_exclusion_correction(s->n, pos, s->q, &s->box, s->excl_off, s->excl_idx, alpha, fn);
         for (size_t k = 0; k < 3 * (size_t)s->n; ++k) f[k] += fn[k];
         free(fn);
-        /* Q6: the reciprocal energy is assigned over the accumulated PE */
+        /* the reciprocal energy is assigned over the accumulated PE (NonbondedForce.cpp:387) */
         e->reference_pe = e->recip + e->self + (e->coulomb + e->lj) - e->exclusion;
         e->potential += e->recip + e->self + e->coulomb + e->lj - e->exclusion;
     }

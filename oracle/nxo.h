@@ -53,9 +53,9 @@ typedef struct {
     const double* q; const double* sig; const double* eps;
     const int* excl_off; const int* excl_idx;     /* symmetric CSR */
     double cutoff, ewald_tol;
-    int grid[3];                 /* PME grid; 0 = reference rule (Q2) */
+    int grid[3];                 /* PME grid; 0 = reference rule (next pow2 >= 10*ceil(L)) */
     int use_switch; double switch_distance;
-    int switch_quirk;            /* 1 = reproduce the reference's carry-over (Q5) */
+    int switch_quirk;            /* 1 = reproduce the reference's switch carry-over between pairs */
     nxo_box box;
 } nxo_system;
 
@@ -63,7 +63,7 @@ typedef struct {
 typedef struct {
     double bond, angle, torsion, coulomb, lj, recip, self, exclusion;
     double potential;      /* physical sum of all enabled terms */
-    double reference_pe;   /* what Engine reports (Q6: recip overwrites bonded) */
+    double reference_pe;   /* what Engine reports (recip energy assigned over the bonded sum) */
 } nxo_energies;
 
 enum { NXO_BOND = 1, NXO_ANGLE = 2, NXO_TORSION = 4, NXO_NONBONDED = 8 };

@@ -32,7 +32,7 @@ def available() -> bool:
 
 
 def box(pos):
-    """PeriodicBoundaryCondition::extractBoxBoundaries (Q1)."""
+    """PeriodicBoundaryCondition::extractBoxBoundaries: bbox of the positions +- 1%."""
     pos = _f64(pos)
     lb, ub, L = np.zeros(3), np.zeros(3), np.zeros(3)
     O.ref().nxr_box_from_positions(pos.shape[0], _d(pos), _d(lb), _d(ub), _d(L))
@@ -183,7 +183,7 @@ def forces(sys, pos=None, excl=None, grid=None, mask=O.ALL):
     """Engine::CalculateForces composed from the reference's functions.
 
     Returns (F, energies) with energies = dict(bond, angle, torsion, recip, self,
-    direct, exclusion, reference_pe); reference_pe follows Q6: with nonbonded on,
+    direct, exclusion, reference_pe); reference_pe is what Engine reports: with nonbonded on,
     the reciprocal energy overwrites the bonded sum (NonbondedForce.cpp:387)."""
     pos = _f64(sys.pos if pos is None else pos)
     excl = exclusions(sys.n, sys.bond_idx) if excl is None else excl
@@ -240,3 +240,25 @@ def simulate(sys, nsteps, dt, interval=1, mask=O.ALL, grid=None):
             rows.append((ke, e["reference_pe"], ke + e["reference_pe"]))
         pos, vel = leapfrog(pos, vel, F, sys.mass, dt, sys.lb, sys.ub)
     return pos, vel, F, np.array(rows)
+
+
+def report(out_name, pos, vel, f, ke, pe, step):
+    """Reporter::TotalEnergyReport + TestPVFReport; files land in the working directory."""
+    import ctypes as C
+    pos, vel, f = _f64(pos), _f64(vel), _f64(f)
+    O.ref().nxr_report(out_name.encode(), C.c_int(pos.shape[0]), _d(pos), _d(vel), _d(f), C.c_double(ke),
+                       C.c_double(pe), C.c_int(step))
+
+
+def pdb_report(in_pdb, out_pdb, pos, model, first):
+    """Reporter::pdbOutputGeneratorPart1 (when first) and Part2."""
+    pos = _f64(pos)
+    O.ref().nxr_pdb_report(in_pdb.encode(), out_pdb.encode(), pos.shape[0], _d(pos), int(model), int(bool(first)))
+
+
+def parse_counts(system_xml, state_xml):
+    """(masses, bonds, angles, torsions, nonbonded particles, positions) as the reference's parsers count them."""
+    import ctypes as C
+    c = (C.c_int * 6)()
+    O.ref().nxr_parse_counts(system_xml.encode(), state_xml.encode() if state_xml else None, c)
+    return tuple(c[:])

@@ -14,10 +14,14 @@ Fixtures (numpy .npz, no pickled objects):
                  box rule, exclusions, sorted pair set, alpha, grid rule,
                  B-spline moduli and theta/dtheta of 16 atoms, spread grid of
                  12 atoms, per-term forces and energies, reciprocal forces at
-                 the grid rule's 64^3, 10 leapfrog steps (dt 1 fs), and
-                 per-term forces and energies at the end of those steps
+                 the grid rule's 64^3, 10 leapfrog steps (dt 1 fs),
+                 per-term forces and energies at the end of those steps, and
+                 the report rows and step-9 state of the same system run as
+                 the dispatcher runs it (box and grid rules, v(0) = 0)
   water3000.npz  1,000 flexible waters (3.107 nm box, grid 32x32x30):
                  per-term forces and energies, pair count and pair-set digest
+  reports.json   sha256 digests of the reference's PDB / TotalEnergy / PVF
+                 files for fixed inputs (the byte formats of its Reporter)
   edge.npz       tiny cases: a pair at exactly r = rc, near-collinear
                  torsions, a near-linear angle, atoms outside [lb, ub] at the
                  leapfrog wrap, and a box shorter than 2 rc
@@ -106,6 +110,17 @@ def chain1500():
     end = s.copy()
     end.pos = pos
     out.update(term_outputs(end, excl, prefix="end_"))
+    # what TaskDispatcher::Simulate reports for this system read from XML: box by
+    # the positions rule, grid by the grid rule, zero initial velocities
+    # (Engine.cpp:53-69, Initializer.cpp:21), all forces, dt 1 fs, interval 1
+    d = s.copy()
+    d.lb, d.ub = lb, ub
+    d.grid = tuple(int(g) for g in R.grid_rule(lb, ub))
+    d.vel = np.zeros_like(s.vel)
+    _, _, _, rows = R.simulate(d, 10, 0.001, interval=1)
+    p9, v9, _, _ = R.simulate(d, 9, 0.001, interval=1)
+    f9, _ = R.forces(d, p9, excl=excl)
+    out.update(dispatch_rows=rows, dispatch_pos9=p9, dispatch_vel9=v9, dispatch_f9=f9)
     return out
 
 
@@ -177,6 +192,46 @@ def _tiny(pos, lb, ub, d):
     return t
 
 
+def reports():
+    """sha256 digests of the reference's report files (Reporter.cpp) for fixed inputs:
+    the Sample_Protein PDB template with the positions of its state XML (models 1
+    and 2, the REMARK date replaced by a token), and the TotalEnergy / PVF files
+    of tests/_util.report_inputs() over steps 0, 1, 2."""
+    import json
+    import tempfile
+    import xml.etree.ElementTree as ET
+
+    sys.path.insert(0, os.path.dirname(HERE))
+    from _util import gunzip_to, normalize_remark_date, report_inputs
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        pdb = gunzip_to(os.path.join(HERE, "Sample_Protein.pdb.gz"), os.path.join(tmp, "in.pdb"))
+        st = gunzip_to(os.path.join(HERE, "state_Sample_Protein.xml.gz"), os.path.join(tmp, "st.xml"))
+        pos = np.array([[float(p.get(k)) for k in "xyz"] for p in ET.parse(st).getroot().find("Positions")])
+        outp = os.path.join(tmp, "out.pdb")
+        R.pdb_report(pdb, outp, pos, 1, True)
+        R.pdb_report(pdb, outp, pos * 1.01 - 0.05, 2, False)
+        data = normalize_remark_date(open(outp, "rb").read())
+        out["sample_protein_pdb_sha256"] = hashlib.sha256(data).hexdigest()
+        out["sample_protein_pdb_bytes"] = len(data)
+        p, v, f, rows = report_inputs()
+        cwd = os.getcwd()
+        os.chdir(tmp)
+        try:
+            for step, ke, pe in rows:
+                R.report("rep.pdb", p * (1 + step), v, f, ke, pe, step)
+        finally:
+            os.chdir(cwd)
+        for name in ("TotalEnergyrep.pdb", "PVFReportrep.pdb"):
+            data = open(os.path.join(tmp, name), "rb").read()
+            out[name + "_sha256"] = hashlib.sha256(data).hexdigest()
+            out[name + "_bytes"] = len(data)
+    path = os.path.join(HERE, "reports.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    print("wrote %s" % path)
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libnxref.so is not built: run `make -C oracle ref` where the reference exists")
@@ -185,6 +240,7 @@ def main():
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **data)
         print("wrote %s (%d arrays, %.0f KB)" % (path, len(data), os.path.getsize(path) / 1024))
+    reports()
 
 
 if __name__ == "__main__":
