@@ -252,6 +252,23 @@ def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, wor
     }
 
 
+PROFILE = os.path.join(ROOT, "profiles", "kernels_solvated_chain_100000.md")
+
+
+def profiled_traffic(kernel="pair_kernel<mixed, false>"):
+    """HBM bytes per launch of `kernel` as recorded by the two rocprofv3 --pmc
+    passes (FETCH_SIZE, WRITE_SIZE, scaled by a 256 MiB copy) of this bench
+    command, from the committed summary; None when it is absent."""
+    try:
+        for line in open(PROFILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[0].endswith(kernel):
+                return float(cells[3]) * 1e6
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def copy_calibration(mib=256, reps=3):
     """A plain copy kernel of known size: 256 MiB read and 256 MiB written per
     launch, to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -357,7 +374,10 @@ def main():
         "peak": PEAK_FP32_VALU,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_VALU,
-        "traffic": None,
+        "traffic": profiled_traffic(),
+        "traffic_source": "profiles/kernels_solvated_chain_100000.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                          "scaled by a 256 MiB copy)",
+        "peak_without_packed_fp32": PEAK_FP32_VALU / 2,
         "algorithmic_flop_per_launch": flops,
         "pairs_within_cutoff": pairs,
         "pairs_evaluated": tiles * 64,
