@@ -389,6 +389,7 @@ class Engine : public EngineBase {
             HostSystem& h = *hs_[s];
             SysDev& d = B.sys[s];
             const int a0 = d.atom_off;
+            const int excl_begin = (int)epairs.size();
             for (int i = 0; i < h.n; ++i) {
                 mass.push_back(h.mass[i]);
                 invm.push_back(h.mass[i] == 0.0 ? 0.0 : 1.0 / h.mass[i]);
@@ -403,8 +404,8 @@ class Engine : public EngineBase {
                 }
                 eoff.push_back((int)eidx.size());
             }
-            d.excl_off = 0;
-            d.n_excl = 0;
+            d.excl_off = excl_begin;
+            d.n_excl = (int)epairs.size() - excl_begin;
             for (int i = 0; i < d.slot_cap; ++i) slot_sys.push_back(s);
             for (int i = 0; i < d.ncx * d.ncy; ++i) col_sys.push_back(s);
             for (int i = 0; i < d.super_cap; ++i) super_sys.push_back(s);

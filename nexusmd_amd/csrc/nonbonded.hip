@@ -358,57 +358,70 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
     }
 }
 
-// Exclusion correction over excluded pairs (i < j), forces in atom order.
+// Exclusion correction (NonbondedForce.cpp:590-630) as a per-atom gather: every
+// atom adds the correction of each excluded partner from the symmetric CSR
+// lists, so each pair is evaluated from both sides and no force is added
+// atomically; the energy of a pair is counted from its lower-index atom.
 template <class P, bool ENERGY>
 __global__ void exclusion_kernel(const SysDev* __restrict__ sd, const int* __restrict__ atom_sys,
-                                 const int2* __restrict__ pairs, int n_pairs, const double* __restrict__ X,
-                                 const double* __restrict__ Y, const double* __restrict__ Z,
-                                 const double* __restrict__ Q, typename P::facc* FX, typename P::facc* FY,
-                                 typename P::facc* FZ, long long* energy) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                 const int* __restrict__ excl_off, const int* __restrict__ excl_idx,
+                                 const double* __restrict__ X, const double* __restrict__ Y,
+                                 const double* __restrict__ Z, const double* __restrict__ Q, typename P::facc* FX,
+                                 typename P::facc* FY, typename P::facc* FZ, long long* energy, int n_atoms) {
+    using real = typename P::real;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
     double eacc = 0.0;
-    int s = -1;
-    if (t < n_pairs) {
-        const int2 pr = pairs[t];
-        const int i = pr.x, j = pr.y;
-        s = atom_sys[i];
+    int s = 0;
+    const bool valid = a < n_atoms;
+    if (valid) {
+        s = atom_sys[a];
         const SysDev& d = sd[s];
-        const double dxd = min_image1(X[i] - X[j], d.L[0]);
-        const double dyd = min_image1(Y[i] - Y[j], d.L[1]);
-        const double dzd = min_image1(Z[i] - Z[j], d.L[2]);
-        const double qq = kOne4PiEps0 * Q[i] * Q[j];
-        if constexpr (P::mixed) {
-            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            const float ir = 1.0f / r;
-            const float al = (float)d.alpha;
-            const float ar = al * r;
-            const float ef = erff(ar);
-            const float qf = (float)qq;
-            if (ef > 1e-6f) {
-                const float dEdR = qf * ir * ir * ir * (ef - 2.0f * ar * __expf(-ar * ar) * (float)(1.0 / kSqrtPi));
-                facc_add(&FX[i], -dx * dEdR); facc_add(&FY[i], -dy * dEdR); facc_add(&FZ[i], -dz * dEdR);
-                facc_add(&FX[j], dx * dEdR); facc_add(&FY[j], dy * dEdR); facc_add(&FZ[j], dz * dEdR);
-                eacc = (double)(qf * ir * ef);
+        const double xa = X[a], ya = Y[a], za = Z[a], qa = kOne4PiEps0 * Q[a];
+        real fx = 0, fy = 0, fz = 0;
+        const int e1 = excl_off[a + 1];
+        for (int e = excl_off[a]; e < e1; ++e) {
+            const int j = excl_idx[e];
+            const double dxd = min_image1(xa - X[j], d.L[0]);
+            const double dyd = min_image1(ya - Y[j], d.L[1]);
+            const double dzd = min_image1(za - Z[j], d.L[2]);
+            const double qq = qa * Q[j];
+            double ej;
+            if constexpr (P::mixed) {
+                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                const float ir = 1.0f / r;
+                const float al = (float)d.alpha;
+                const float ar = al * r;
+                const float ef = erff(ar);
+                const float qf = (float)qq;
+                if (ef > 1e-6f) {
+                    const float dEdR = qf * ir * ir * ir * (ef - 2.0f * ar * __expf(-ar * ar) * (float)(1.0 / kSqrtPi));
+                    fx -= dx * dEdR; fy -= dy * dEdR; fz -= dz * dEdR;
+                    ej = (double)(qf * ir * ef);
+                } else {
+                    ej = d.alpha * kTwoOverSqrtPi * qq;
+                }
             } else {
-                eacc = d.alpha * kTwoOverSqrtPi * qq;
+                const double r = sqrt(dxd * dxd + dyd * dyd + dzd * dzd);
+                const double ir = 1.0 / r;
+                const double ar = d.alpha * r;
+                const double ef = erf(ar);
+                if (ef > 1e-6) {
+                    const double dEdR = qq * ir * ir * ir * (ef - 2.0 * ar * exp(-ar * ar) / kSqrtPi);
+                    fx -= dxd * dEdR; fy -= dyd * dEdR; fz -= dzd * dEdR;
+                    ej = qq * ir * ef;
+                } else {
+                    ej = d.alpha * kTwoOverSqrtPi * qq;
+                }
             }
-        } else {
-            const double r = sqrt(dxd * dxd + dyd * dyd + dzd * dzd);
-            const double ir = 1.0 / r;
-            const double ar = d.alpha * r;
-            const double ef = erf(ar);
-            if (ef > 1e-6) {
-                const double dEdR = qq * ir * ir * ir * (ef - 2.0 * ar * exp(-ar * ar) / kSqrtPi);
-                facc_add(&FX[i], -dxd * dEdR); facc_add(&FY[i], -dyd * dEdR); facc_add(&FZ[i], -dzd * dEdR);
-                facc_add(&FX[j], dxd * dEdR); facc_add(&FY[j], dyd * dEdR); facc_add(&FZ[j], dzd * dEdR);
-                eacc = qq * ir * ef;
-            } else {
-                eacc = d.alpha * kTwoOverSqrtPi * qq;
-            }
+            if (j > a) eacc += ej;
         }
+        // the bonded kernels before this one have finished; one thread per atom
+        facc_set_add(&FX[a], fx);
+        facc_set_add(&FY[a], fy);
+        facc_set_add(&FZ[a], fz);
     }
-    if constexpr (ENERGY) wave_energy_add(energy, s < 0 ? 0 : s, E_EXCL, eacc, s >= 0);
+    if constexpr (ENERGY) wave_energy_add(energy, s, E_EXCL, eacc, valid);
 }
 
 // Explicit pair set (atom ids, first > second) with the FP64 test of the pair kernel.
@@ -467,13 +480,13 @@ template <class P>
 void launch_exclusion(Batch<P>& B, bool energy) {
     if (B.n_excl_pairs == 0) return;
     const int T = 256;
-    const unsigned nb = (unsigned)((B.n_excl_pairs + T - 1) / T);
+    const unsigned nb = (unsigned)((B.n_atoms + T - 1) / T);
     if (energy)
-        exclusion_kernel<P, true><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_pairs, B.n_excl_pairs, B.X, B.Y,
-                                                         B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy);
+        exclusion_kernel<P, true><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_off, B.excl_idx, B.X, B.Y,
+                                                         B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy, B.n_atoms);
     else
-        exclusion_kernel<P, false><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_pairs, B.n_excl_pairs, B.X,
-                                                          B.Y, B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy);
+        exclusion_kernel<P, false><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_off, B.excl_idx, B.X, B.Y,
+                                                          B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy, B.n_atoms);
 }
 
 template <class P>

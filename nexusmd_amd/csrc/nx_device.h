@@ -102,6 +102,9 @@ __device__ __forceinline__ void facc_add(long long* p, float v) {
 __device__ __forceinline__ void facc_add(long long* p, double v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(to_fixed(v)));
 }
+// Non-atomic add, for a location that no other thread writes meanwhile.
+__device__ __forceinline__ void facc_set_add(double* p, double v) { *p += v; }
+__device__ __forceinline__ void facc_set_add(long long* p, float v) { *p += to_fixed(v); }
 __device__ __forceinline__ double facc_get(const double* p) { return *p; }
 __device__ __forceinline__ double facc_get(const long long* p) { return static_cast<double>(*p) * (1.0 / kForceScale); }
 

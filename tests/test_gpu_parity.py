@@ -153,3 +153,38 @@ def test_graph_steps_match_plain(chain, prec):
         assert np.allclose(pa, pb, rtol=0, atol=1e-9)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_exclusions_outside_the_pair_list(prec):
+    """Bonded partners farther apart than the list radius: their exclusion
+    correction cannot come from list tiles, so the system falls back to the
+    per-pair exclusion kernel.  Forces and energies as the oracle's."""
+    rng = np.random.default_rng(21)
+    n, L_ = 120, 4.0
+    pos = rng.uniform(0, L_, (n, 3))
+    q = rng.normal(0, 0.5, n)
+    q -= q.mean()
+    from _util import bare_system
+    s = bare_system(pos, [0, 0, 0], [L_, L_, L_], q=q, sig=rng.uniform(0.25, 0.35, n),
+                    eps=rng.uniform(0.1, 1.0, n), grid=(32, 32, 32))
+    # spread atoms apart so LJ stays moderate, then bond a few far-apart pairs
+    g = np.array([[x, y, z] for x in range(5) for y in range(5) for z in range(5)])[:n] * 0.8 + 0.1
+    s.pos = g + rng.normal(0, 0.02, g.shape)
+    far = []
+    for i in range(0, 40, 2):
+        j = (i + 37) % n
+        d = s.pos[i] - s.pos[j]
+        d -= L_ * np.round(d / L_)
+        if np.linalg.norm(d) > 1.3:
+            far.append((i, j))
+    assert len(far) >= 5
+    s.bond_idx = np.array(far, np.int32)
+    s.bond_par = np.tile([0.15, 1000.0], (len(far), 1))
+    ctx = _ctx(s, prec, skin=0.1)
+    f, e = ctx.compute_forces(L.NONBONDED)
+    fr, er = O.compute_forces(O.Prepared(s), mask=O.NONBONDED)
+    tol = FP64_TOL if prec == L.FP64 else MIXED_TOL
+    assert rel_rms(f, fr) <= tol
+    assert abs(e["exclusion"] - er["exclusion"]) <= (1e-9 if prec == L.FP64 else 1e-5) * abs(er["exclusion"])
+    ctx.close()
