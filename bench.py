@@ -72,13 +72,15 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
     r = prec_bytes
     pos = 4 * r                                   # x, y, z, q
     nterms = len(sysm.bond_idx) * 2 + len(sysm.angle_idx) * 3 + len(sysm.torsion_idx) * 4
+    nslots = nterms
     return {
         "pme_spread": N * pos + G * 8,
         "fft": 2 * (G * r + Gc * 2 * r),
         "pme_gather": N * pos + G * r + N * 24,
         "integrate": N * 128,
-        "bonded": (len(sysm.bond_idx) * 24 + len(sysm.angle_idx) * 32 + len(sysm.torsion_idx) * 48) + nterms * 48,
-        "exclusion": n_excl * (8 + 2 * 32 + 2 * 24),
+        "bonded": (len(sysm.bond_idx) * 24 + len(sysm.angle_idx) * 32 + len(sysm.torsion_idx) * 48)
+                  + nterms * (24 + 3 * prec_bytes),
+        "atom_forces": N * (8 + 24) + 2 * n_excl * (4 + 24 + 8) + nslots * (4 + 3 * prec_bytes) + N * 24,
     }
 
 
@@ -113,7 +115,7 @@ def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
     nb = min(steps, breakdown_steps)
     ctx.step(nb, sysm.dt)
     kern = {}
-    for name in ("pair", "pme_spread", "fft", "pme_gather", "bonded", "exclusion", "integrate", "nlist", "pack"):
+    for name in ("pair", "pme_spread", "fft", "pme_gather", "bonded", "atom_forces", "integrate", "nlist", "pack"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"ms_per_launch": ms, "launches": n}
     ctx.enable_timing(L.TIMING_OFF)

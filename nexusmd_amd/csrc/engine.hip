@@ -474,6 +474,33 @@ class Engine : public EngineBase {
             for (int i = 0; i < h.n; ++i) se -= kOne4PiEps0 * h.q[i] * h.q[i] * d.alpha / kSqrtPi;
             self_energy_[s] = se;
         }
+        // term-force slots: bonds 2, angles 3, torsions 4 per term, in that order;
+        // per atom the slots acting on it (CSR, ascending slot order)
+        {
+            const int nb = (int)bidx.size(), na = (int)aidx.size(), nt = (int)tidx.size();
+            B.n_tslots = 2 * nb + 3 * na + 4 * nt;
+            std::vector<int> cnt(B.n_atoms + 1, 0);
+            auto each = [&](auto&& fn) {
+                for (int t = 0; t < nb; ++t) { fn(bidx[t].x, 2 * t); fn(bidx[t].y, 2 * t + 1); }
+                for (int t = 0; t < na; ++t) {
+                    const int b = 2 * nb + 3 * t;
+                    fn(aidx[t].x, b); fn(aidx[t].y, b + 1); fn(aidx[t].z, b + 2);
+                }
+                for (int t = 0; t < nt; ++t) {
+                    const int b = 2 * nb + 3 * na + 4 * t;
+                    fn(tidx[t].x, b); fn(tidx[t].y, b + 1); fn(tidx[t].z, b + 2); fn(tidx[t].w, b + 3);
+                }
+            };
+            each([&](int a, int) { ++cnt[a + 1]; });
+            for (int a = 0; a < B.n_atoms; ++a) cnt[a + 1] += cnt[a];
+            std::vector<int> at_idx(std::max(1, B.n_tslots)), fill(cnt.begin(), cnt.end() - 1);
+            each([&](int a, int slot) { at_idx[fill[a]++] = slot; });
+            dalloc(&B.at_off, cnt.size(), owned_);
+            dalloc(&B.at_idx, at_idx.size(), owned_);
+            dalloc(&B.tF, (size_t)3 * std::max(1, B.n_tslots), owned_);
+            upload(B.at_off, cnt, st);
+            upload(B.at_idx, at_idx, st);
+        }
         B.n_excl_pairs = (int)epairs.size();
         dalloc(&B.excl_off, eoff.size(), owned_);
         dalloc(&B.excl_idx, std::max<size_t>(1, eidx.size()), owned_);
@@ -620,7 +647,7 @@ class Engine : public EngineBase {
             timed("nlist", [&] { launch_nlist<P>(B); });
             timed("pack", [&] { launch_pack<P>(B); });
             if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
-            if (mask & NX_FORCE_NONBONDED) timed("exclusion", [&] { launch_exclusion<P>(B, energy); });
+            timed("atom_forces", [&] { launch_atom_forces<P>(B, mask, energy); });
         }
         if (!(mask & NX_FORCE_NONBONDED)) return;
         if (part < 0 || part == 1) timed("pair", [&] { launch_pair<P>(B, energy); });
@@ -787,10 +814,6 @@ class Engine : public EngineBase {
         if (integrate) {
             timed("integrate", [&] { launch_integrate<P>(B, dt); });
             ++steps_;
-        } else {
-            HIPCHK(hipMemsetAsync(B.FAx, 0, sizeof(facc) * B.n_atoms, B.stream));
-            HIPCHK(hipMemsetAsync(B.FAy, 0, sizeof(facc) * B.n_atoms, B.stream));
-            HIPCHK(hipMemsetAsync(B.FAz, 0, sizeof(facc) * B.n_atoms, B.stream));
         }
         end_call();
     }

@@ -122,9 +122,6 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
         const double dz = min_image1(z - ZB[a], d.L[2]);
         const double d2 = dx * dx + dy * dy + dz * dz;
         moved = !(d2 <= d.skin_half2);
-        FAx[a] = 0;
-        FAy[a] = 0;
-        FAz[a] = 0;
     }
     // lanes of a wave storing to one flag merge into one write
     if (moved) sc[s].rebuild = 1;

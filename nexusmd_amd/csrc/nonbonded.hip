@@ -358,72 +358,6 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
     }
 }
 
-// Exclusion correction (NonbondedForce.cpp:590-630) as a per-atom gather: every
-// atom adds the correction of each excluded partner from the symmetric CSR
-// lists, so each pair is evaluated from both sides and no force is added
-// atomically; the energy of a pair is counted from its lower-index atom.
-template <class P, bool ENERGY>
-__global__ void exclusion_kernel(const SysDev* __restrict__ sd, const int* __restrict__ atom_sys,
-                                 const int* __restrict__ excl_off, const int* __restrict__ excl_idx,
-                                 const double* __restrict__ X, const double* __restrict__ Y,
-                                 const double* __restrict__ Z, const double* __restrict__ Q, typename P::facc* FX,
-                                 typename P::facc* FY, typename P::facc* FZ, long long* energy, int n_atoms) {
-    using real = typename P::real;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    double eacc = 0.0;
-    int s = 0;
-    const bool valid = a < n_atoms;
-    if (valid) {
-        s = atom_sys[a];
-        const SysDev& d = sd[s];
-        const double xa = X[a], ya = Y[a], za = Z[a], qa = kOne4PiEps0 * Q[a];
-        real fx = 0, fy = 0, fz = 0;
-        const int e1 = excl_off[a + 1];
-        for (int e = excl_off[a]; e < e1; ++e) {
-            const int j = excl_idx[e];
-            const double dxd = min_image1(xa - X[j], d.L[0]);
-            const double dyd = min_image1(ya - Y[j], d.L[1]);
-            const double dzd = min_image1(za - Z[j], d.L[2]);
-            const double qq = qa * Q[j];
-            double ej;
-            if constexpr (P::mixed) {
-                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-                const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-                const float ir = 1.0f / r;
-                const float al = (float)d.alpha;
-                const float ar = al * r;
-                const float ef = erff(ar);
-                const float qf = (float)qq;
-                if (ef > 1e-6f) {
-                    const float dEdR = qf * ir * ir * ir * (ef - 2.0f * ar * __expf(-ar * ar) * (float)(1.0 / kSqrtPi));
-                    fx -= dx * dEdR; fy -= dy * dEdR; fz -= dz * dEdR;
-                    ej = (double)(qf * ir * ef);
-                } else {
-                    ej = d.alpha * kTwoOverSqrtPi * qq;
-                }
-            } else {
-                const double r = sqrt(dxd * dxd + dyd * dyd + dzd * dzd);
-                const double ir = 1.0 / r;
-                const double ar = d.alpha * r;
-                const double ef = erf(ar);
-                if (ef > 1e-6) {
-                    const double dEdR = qq * ir * ir * ir * (ef - 2.0 * ar * exp(-ar * ar) / kSqrtPi);
-                    fx -= dxd * dEdR; fy -= dyd * dEdR; fz -= dzd * dEdR;
-                    ej = qq * ir * ef;
-                } else {
-                    ej = d.alpha * kTwoOverSqrtPi * qq;
-                }
-            }
-            if (j > a) eacc += ej;
-        }
-        // the bonded kernels before this one have finished; one thread per atom
-        facc_set_add(&FX[a], fx);
-        facc_set_add(&FY[a], fy);
-        facc_set_add(&FZ[a], fz);
-    }
-    if constexpr (ENERGY) wave_energy_add(energy, s, E_EXCL, eacc, valid);
-}
-
 // Explicit pair set (atom ids, first > second) with the FP64 test of the pair kernel.
 template <class P>
 __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, const SysCount* sc, const int* super_sys,
@@ -477,19 +411,6 @@ void launch_pair(Batch<P>& B, bool energy) {
 }
 
 template <class P>
-void launch_exclusion(Batch<P>& B, bool energy) {
-    if (B.n_excl_pairs == 0) return;
-    const int T = 256;
-    const unsigned nb = (unsigned)((B.n_atoms + T - 1) / T);
-    if (energy)
-        exclusion_kernel<P, true><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_off, B.excl_idx, B.X, B.Y,
-                                                         B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy, B.n_atoms);
-    else
-        exclusion_kernel<P, false><<<nb, T, 0, B.stream>>>(B.d_sys, B.atom_sys, B.excl_off, B.excl_idx, B.X, B.Y,
-                                                          B.Z, B.q, B.FAx, B.FAy, B.FAz, B.energy, B.n_atoms);
-}
-
-template <class P>
 long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count) {
     hipMemsetAsync(d_count, 0, sizeof(long long), B.stream);
     export_pairs_kernel<P><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.list,
@@ -504,8 +425,6 @@ long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* 
 
 template void launch_pair<PrecFP64>(Batch<PrecFP64>&, bool);
 template void launch_pair<PrecMixed>(Batch<PrecMixed>&, bool);
-template void launch_exclusion<PrecFP64>(Batch<PrecFP64>&, bool);
-template void launch_exclusion<PrecMixed>(Batch<PrecMixed>&, bool);
 template long long launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);
 template long long launch_export_pairs<PrecMixed>(Batch<PrecMixed>&, int2*, long long, long long*);
 

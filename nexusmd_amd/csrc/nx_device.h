@@ -33,6 +33,8 @@ constexpr int kClusterSize = 8;
 constexpr int kSuperClusters = 8;                           // clusters per super-cluster
 constexpr int kPmeOrder = 5;
 
+constexpr int NX_FORCE_NONBONDED_BIT = 8;   // the nonbonded bit of a force mask
+
 enum EnergyTerm { E_BOND = 0, E_ANGLE, E_TORSION, E_COUL, E_LJ, E_RECIP, E_EXCL, E_KIN, E_NTERMS };
 
 // Per-system constants (device copy; written once per topology/box).
@@ -102,9 +104,11 @@ __device__ __forceinline__ void facc_add(long long* p, float v) {
 __device__ __forceinline__ void facc_add(long long* p, double v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(to_fixed(v)));
 }
-// Non-atomic add, for a location that no other thread writes meanwhile.
+// Non-atomic add / store, for a location that no other thread writes meanwhile.
 __device__ __forceinline__ void facc_set_add(double* p, double v) { *p += v; }
 __device__ __forceinline__ void facc_set_add(long long* p, float v) { *p += to_fixed(v); }
+__device__ __forceinline__ void facc_set(double* p, double v) { *p = v; }
+__device__ __forceinline__ void facc_set(long long* p, float v) { *p = to_fixed(v); }
 __device__ __forceinline__ double facc_get(const double* p) { return *p; }
 __device__ __forceinline__ double facc_get(const long long* p) { return static_cast<double>(*p) * (1.0 / kForceScale); }
 

@@ -69,6 +69,10 @@ struct Batch {
     int4* angle_idx = nullptr;
     double2* angle_par = nullptr;
     int n_angles = 0;
+    real* tF = nullptr;             // per-term forces [3][n_tslots]: bonds 2, angles 3, torsions 4 slots each
+    int n_tslots = 0;
+    int* at_off = nullptr;          // per atom: CSR of the term-force slots that act on it
+    int* at_idx = nullptr;
     int4* tors_idx = nullptr;
     double4* tors_par = nullptr;   // k, phase, periodicity, unused
     int n_tors = 0;
@@ -100,8 +104,8 @@ struct Batch {
 template <class P> void launch_nlist(Batch<P>& B);
 template <class P> void launch_pack(Batch<P>& B);
 template <class P> void launch_pair(Batch<P>& B, bool energy);
-template <class P> void launch_exclusion(Batch<P>& B, bool energy);
 template <class P> void launch_bonded(Batch<P>& B, int mask, bool energy);
+template <class P> void launch_atom_forces(Batch<P>& B, int mask, bool energy);
 template <class P> void launch_pme_spread(Batch<P>& B);
 template <class P> void launch_fft_forward(Batch<P>& B);
 template <class P> void launch_fft_convolve(Batch<P>& B, bool energy);
