@@ -243,9 +243,9 @@ class Engine : public EngineBase {
             int g[3];
             for (int k = 0; k < 3; ++k) g[k] = h.nb.grid[k] > 0 ? h.nb.grid[k] : grid_rule(d.L[k]);
             for (int k = 0; k < 3; ++k)
-                if (!fft_size_ok(g[k]) || g[k] < 6)
+                if (!fft_size_ok(g[k]) || g[k] < 6 || g[k] > 1024)
                     throw HipError(NX_ERR_ARG, "PME grid dimension " + std::to_string(g[k]) +
-                                                   " must be >= 6 with prime factors 2, 3, 5 only");
+                                                   " must be in [6, 1024] with prime factors 2, 3, 5 only");
             for (int k = 0; k < 3; ++k) d.grid[k] = g[k];
             d.rgrid_off = rg_off;
             d.cgrid_off = cg_off;
@@ -381,7 +381,8 @@ class Engine : public EngineBase {
         std::vector<int4> zi, yi, xi, sp_tiles;
         int direct_slots = 0;
         const int maxK = fft_maxK_;
-        const int T = std::max(1, std::min(16, (int)(61440 / (2 * maxK * (int)sizeof(real2)))));
+        // LDS per FFT block: two buffers of T lines plus the twiddles, within 60 KiB
+        const int T = std::max(1, std::min(16, (int)((61440 / (int)sizeof(real2) - maxK) / (2 * maxK))));
         B.fft_lines = T;
         B.fft_tile = T;
         B.fft_maxK = maxK;

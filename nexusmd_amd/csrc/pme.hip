@@ -364,10 +364,23 @@ __device__ __forceinline__ int next_radix(int m) {
     return 5;
 }
 
-// FFT of `lines` lines of length N stored [line][N] in `a`; `b` is scratch.
-// Returns the buffer holding the result.  All threads of the block call it.
+// Quotient of small non-negative integers (n < 2^20, d < 2^12) through a float
+// reciprocal: (n + 0.5) / d stays at least 0.5 / d away from an integer, far
+// beyond the rounding of the product.
+__device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
+
+// Copies the N twiddles of an axis into LDS (all threads of the block).
+template <typename R2>
+__device__ __forceinline__ void load_twiddles(R2* dst, const R2* __restrict__ src, int N) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+}
+
+// FFT of `lines` lines of length N stored [line][N] in `a`; `b` is scratch;
+// `tw` holds exp(-2 pi i m / N), m < N, in LDS.  Stockham autosort, radix 4,
+// 2, 3, 5; unnormalised; sign -1 forward, +1 backward.  Returns the buffer
+// holding the result.  All threads of the block call it.
 template <typename real, typename R2>
-__device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* __restrict__ tw, int sign) {
+__device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* tw, int sign) {
     int Ns = 1;
     int rem = N;
     R2* src = a;
@@ -376,11 +389,13 @@ __device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* __restric
         const int r = next_radix(rem);
         const int nb = N / r;
         const int step = N / (Ns * r);
-        for (int w = threadIdx.x; w < lines * nb; w += blockDim.x) {
-            const int line = w / nb, j = w - line * nb;
+        const float inv_nb = 1.0f / (float)nb, inv_ns = 1.0f / (float)Ns;
+        const int total = lines * nb;
+        for (int w = threadIdx.x; w < total; w += blockDim.x) {
+            const int line = qdiv(w, inv_nb), j = w - line * nb;
+            const int jq = qdiv(j, inv_ns), k = j - jq * Ns;
             const R2* sp = src + line * N;
             R2* dp = dst + line * N;
-            const int k = j % Ns;
             R2 v[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
@@ -389,14 +404,14 @@ __device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* __restric
 #pragma unroll
                 for (int q = 1; q < 5; ++q) {
                     if (q < r) {
-                        R2 t = tw[(k * q * step) % N];
+                        R2 t = tw[k * q * step];     // k q step < N
                         if (sign > 0) t.y = -t.y;
                         v[q] = cmul(v[q], t);
                     }
                 }
             }
             dft_small<real, R2>(v, r, sign);
-            const int base = (j / Ns) * Ns * r + k;
+            const int base = jq * Ns * r + k;
 #pragma unroll
             for (int q = 0; q < 5; ++q)
                 if (q < r) dp[base + q * Ns] = v[q];
@@ -429,6 +444,8 @@ __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, 
     const int line0 = it.y, nl = it.z;
     R2* A = smem;
     R2* B = smem + nl * Kz;
+    R2* TW = B + nl * Kz;
+    load_twiddles(TW, tw + d.tw_off[2], Kz);
     const typename P::facc* g = rgrid + d.rgrid_off + (long long)line0 * Kz;
     for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) {
         R2 v;
@@ -437,10 +454,11 @@ __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, 
         A[i] = v;
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, tw + d.tw_off[2], -1);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, TW, -1);
     R2* c = cgrid + d.cgrid_off + (long long)line0 * Kzh;
+    const float inv_kzh = 1.0f / (float)Kzh;
     for (int i = threadIdx.x; i < nl * Kzh; i += blockDim.x) {
-        const int t = i / Kzh, kz = i - t * Kzh;
+        const int t = qdiv(i, inv_kzh), kz = i - t * Kzh;
         c[i] = res[t * Kz + kz];
     }
 }
@@ -460,16 +478,19 @@ __global__ __launch_bounds__(256) void fft_z_inv(const SysDev* __restrict__ sd, 
     const int line0 = it.y, nl = it.z;
     R2* A = smem;
     R2* B = smem + nl * Kz;
+    R2* TW = B + nl * Kz;
+    load_twiddles(TW, tw + d.tw_off[2], Kz);
     const R2* c = cgrid + d.cgrid_off + (long long)line0 * Kzh;
+    const float inv_kz = 1.0f / (float)Kz;
     for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) {
-        const int t = i / Kz, z = i - t * Kz;
+        const int t = qdiv(i, inv_kz), z = i - t * Kz;
         R2 v;
         if (z < Kzh) v = c[t * Kzh + z];
         else { v = c[t * Kzh + (Kz - z)]; v.y = -v.y; }
         A[i] = v;
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, tw + d.tw_off[2], +1);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, TW, +1);
     real* o = phi + d.rgrid_off + (long long)line0 * Kz;
     for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) o[i] = res[i].x;
 }
@@ -490,15 +511,18 @@ __global__ __launch_bounds__(256) void fft_y(const SysDev* __restrict__ sd, cons
     const int nt = min(TZ, Kzh - kz0);
     R2* A = smem;
     R2* B = smem + TZ * Ky;
+    R2* TW = B + TZ * Ky;
+    load_twiddles(TW, tw + d.tw_off[1], Ky);
     R2* c = cgrid + d.cgrid_off + (long long)x * Ky * Kzh + kz0;
+    const float inv_nt = 1.0f / (float)nt;
     for (int i = threadIdx.x; i < nt * Ky; i += blockDim.x) {
-        const int y = i / nt, t = i - y * nt;
+        const int y = qdiv(i, inv_nt), t = i - y * nt;
         A[t * Ky + y] = c[(long long)y * Kzh + t];
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Ky, nt, tw + d.tw_off[1], sign);
+    R2* res = fft_lines_lds<real, R2>(A, B, Ky, nt, TW, sign);
     for (int i = threadIdx.x; i < nt * Ky; i += blockDim.x) {
-        const int y = i / nt, t = i - y * nt;
+        const int y = qdiv(i, inv_nt), t = i - y * nt;
         c[(long long)y * Kzh + t] = res[t * Ky + y];
     }
 }
@@ -524,15 +548,18 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
     const long long sx = (long long)Ky * Kzh;
     R2* c = cgrid + d.cgrid_off + (long long)y * Kzh + kz0;
     const real* et = eta + d.cgrid_off + (long long)y * Kzh + kz0;
+    R2* TW = B + TZ * Kx;
+    load_twiddles(TW, tw + d.tw_off[0], Kx);
+    const float inv_nt = 1.0f / (float)nt;
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
-        const int x = i / nt, t = i - x * nt;
+        const int x = qdiv(i, inv_nt), t = i - x * nt;
         A[t * Kx + x] = c[x * sx + t];
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, tw + d.tw_off[0], -1);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, TW, -1);
     long long e = 0;   // fixed point per element: independent of the tile shape
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
-        const int x = i / nt, t = i - x * nt;
+        const int x = qdiv(i, inv_nt), t = i - x * nt;
         const real w = eta ? et[x * sx + t] : (real)0;
         R2 v = res[t * Kx + x];
         if constexpr (ENERGY) {
@@ -546,9 +573,9 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
     }
     __syncthreads();
     R2* other = (res == A) ? B : A;
-    R2* res2 = fft_lines_lds<real, R2>(res, other, Kx, nt, tw + d.tw_off[0], +1);
+    R2* res2 = fft_lines_lds<real, R2>(res, other, Kx, nt, TW, +1);
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
-        const int x = i / nt, t = i - x * nt;
+        const int x = qdiv(i, inv_nt), t = i - x * nt;
         c[x * sx + t] = res2[t * Kx + x];
     }
     if constexpr (ENERGY) block_energy_add(&energy[s * E_NTERMS + E_RECIP], e);
@@ -572,14 +599,17 @@ __global__ __launch_bounds__(256) void fft_x_plain(const SysDev* __restrict__ sd
     R2* B = smem + TZ * Kx;
     const long long sx = (long long)Ky * Kzh;
     R2* c = cgrid + d.cgrid_off + (long long)y * Kzh + kz0;
+    R2* TW = B + TZ * Kx;
+    load_twiddles(TW, tw + d.tw_off[0], Kx);
+    const float inv_nt = 1.0f / (float)nt;
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
-        const int x = i / nt, t = i - x * nt;
+        const int x = qdiv(i, inv_nt), t = i - x * nt;
         A[t * Kx + x] = c[x * sx + t];
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, tw + d.tw_off[0], sign);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, TW, sign);
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
-        const int x = i / nt, t = i - x * nt;
+        const int x = qdiv(i, inv_nt), t = i - x * nt;
         c[x * sx + t] = res[t * Kx + x];
     }
 }
@@ -609,7 +639,7 @@ void launch_pme_spread(Batch<P>& B) {
 
 template <class P>
 static size_t fft_smem(const Batch<P>& B, int lines, int N) {
-    return (size_t)2 * lines * N * sizeof(typename P::real2);
+    return (size_t)(2 * lines + 1) * N * sizeof(typename P::real2);   // two line buffers + twiddles
 }
 
 template <class P>
@@ -679,7 +709,7 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
             tw[o + t].y = (real)sin(ang);
         }
     const int maxK = std::max(nx, std::max(ny, nz));
-    const int T = std::max(1, std::min(16, (int)(61440 / (2 * maxK * (int)sizeof(R2)))));
+    const int T = std::max(1, std::min(16, (int)((61440 / (int)sizeof(R2) - maxK) / (2 * maxK))));
     std::vector<int4> zi, yi, xi;
     for (int l = 0; l < nx * ny; l += T) zi.push_back(make_int4(0, l, std::min(T, nx * ny - l), 0));
     for (int x = 0; x < nx; ++x)
@@ -701,7 +731,7 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
     hipMemcpyAsync(dz, zi.data(), zi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dy, yi.data(), yi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dx, xi.data(), xi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
-    const size_t sm = (size_t)2 * T * maxK * sizeof(R2);
+    const size_t sm = (size_t)(2 * T + 1) * maxK * sizeof(R2);
     if (forward) {
         std::vector<typename P::facc> h(G);
         for (size_t i = 0; i < G; ++i) {
