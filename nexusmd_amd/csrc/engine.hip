@@ -350,6 +350,7 @@ class Engine : public EngineBase {
         dalloc(&B.super_item, B.n_super_cap, o);
         dalloc(&B.chunk_sys, B.n_chunk_cap, o);
         dalloc(&B.slot_zkey, S, o);
+        dalloc(&B.slot_wpos, S, o);
         dalloc(&B.chunks, B.n_chunk_cap, o);
         dalloc(&B.chunk_prefix, B.nsys + 1, o);
         dalloc(&B.list, (size_t)B.n_list_cap, o);
@@ -382,7 +383,7 @@ class Engine : public EngineBase {
         int direct_slots = 0;
         const int maxK = fft_maxK_;
         // LDS per FFT block: two buffers of T lines plus the twiddles, within 60 KiB
-        const int T = std::max(1, std::min(16, (int)((61440 / (int)sizeof(real2) - maxK) / (2 * maxK))));
+        const int T = std::max(1, std::min(8, (int)((61440 / (int)sizeof(real2) - maxK) / (2 * maxK))));
         B.fft_lines = T;
         B.fft_tile = T;
         B.fft_maxK = maxK;

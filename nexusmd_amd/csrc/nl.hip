@@ -3,17 +3,20 @@
 // Replaces obtainNeighborList / VoxelHash (core/baseline/src/Forces/
 // NeighborList.cpp:12-154), which rebuilds a std::map voxel hash every step.
 // Here, per rebuild:
-//   1. nl_bin      : counting sort of atoms into xy columns (atomic counts);
+//   1. nl_bin      : counting sort of atoms into xy columns (one atomic per
+//                    distinct column in a wavefront);
 //   2. nl_scan     : per-system exclusive scan -> slot ranges (columns padded
 //                    to 8-atom clusters) and super-cluster ranges;
 //   3. nl_scatter  : atoms into their column's slot range;
 //   4. nl_columns  : per column, sort by (z, atom id) in LDS -> deterministic
-//                    slot order; cluster bounding boxes; super-cluster items;
-//   5. nl_pairlist : one wavefront per super-cluster (8 clusters of 8 atoms):
-//                    j-clusters of neighbouring columns whose bounding box is
-//                    within rlist of an i-cluster (half list: j >= i), an
-//                    8-bit mask of interacting i-clusters per entry, and
-//                    64-bit exclusion masks for tiles holding excluded pairs.
+//                    slot order; cluster bounding boxes; build positions in
+//                    slot order;
+//   5. nl_pairlist : one block per super-cluster (8 clusters of 8 atoms):
+//                    j-clusters of nearby columns whose bounding box is within
+//                    rlist of an i-cluster (half list: j >= i), then per tile
+//                    the 64 atom pairs, keeping only tiles with a pair within
+//                    rlist; an 8-bit mask of interacting i-clusters per entry,
+//                    and 64-bit exclusion masks for tiles holding excluded pairs.
 // The pair kernel applies r^2 < rc^2 exactly per pair, so the evaluated pair
 // set is the reference's set whatever the buffer (skin) is.
 //
@@ -50,6 +53,33 @@ __device__ __forceinline__ void wrapped(const SysDev& d, double x, double y, dou
     }
 }
 
+// Adds one to counter[key] for every active lane of the wavefront with one
+// atomic per distinct key (neighbouring atoms mostly share a column, so a
+// wavefront touches few columns); returns the counter's old value plus the
+// lane's rank among the lanes that share its key.
+__device__ __forceinline__ int wave_keyed_increment(int* counter, int key) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    bool done = false;
+    int res = 0;
+    for (;;) {
+        const unsigned long long pending = __ballot(!done);
+        if (pending == 0ull) break;
+        const int leader = __builtin_ctzll(pending);
+        const int k = __shfl(key, leader);
+        const bool mine = !done && key == k;
+        const unsigned long long grp = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counter[k], __popcll(grp));
+        base = __shfl(base, leader);
+        if (mine) {
+            res = base + __popcll(grp & below);
+            done = true;
+        }
+    }
+    return res;
+}
+
 __global__ void nl_bin(const SysDev* sd, SysCount* sc, const int* atom_sys, const double* X, const double* Y,
                        const double* Z, int* atom_col, int* col_count, int n_atoms_total) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,7 +95,7 @@ __global__ void nl_bin(const SysDev* sd, SysCount* sc, const int* atom_sys, cons
     cy = cy < 0 ? 0 : (cy >= d.ncy ? d.ncy - 1 : cy);
     int col = d.col_off + cx * d.ncy + cy;
     atom_col[a] = col;
-    atomicAdd(&col_count[col], 1);
+    wave_keyed_increment(col_count, col);
 }
 
 // One block per system: scan padded column counts.
@@ -124,7 +154,7 @@ __global__ void nl_scatter(const SysDev* sd, SysCount* sc, const int* atom_sys, 
     int s = atom_sys[a];
     if (!sc[s].rebuild || (sc[s].overflow & 1)) return;
     int col = atom_col[a];
-    int k = atomicAdd(&col_fill[col], 1);
+    int k = wave_keyed_increment(col_fill, col);
     int slot = col_slot[col] + k;
     if (slot < sd[s].slot_off + sd[s].slot_cap) slot_atom_tmp[slot] = a;
 }
@@ -135,7 +165,7 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
                                                   const int* slot_atom_tmp, const double* X, const double* Y,
                                                   const double* Z, int* atom_of_slot, int* slot_of_atom,
                                                   float4* cl_ctr, float4* cl_half, int4* super_item,
-                                                  float* slot_zkey) {
+                                                  float* slot_zkey, float4* slot_wpos) {
     const int col = blockIdx.x;
     const int s = col_sys[col];
     if (!sc[s].rebuild || (sc[s].overflow & 1)) return;
@@ -184,14 +214,20 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
     for (int i = threadIdx.x; i < padded; i += blockDim.x) {
         int slot = first + i;
         if (i < cnt) {
-            atom_of_slot[slot] = ids[i];
-            slot_of_atom[ids[i]] = slot;
+            const int a = ids[i];
+            atom_of_slot[slot] = a;
+            slot_of_atom[a] = slot;
             // round down so that a key never exceeds the true z (slab binning stays monotone)
             float zk = (float)key[i];
             if ((double)zk > key[i]) zk = nextafterf(zk, -1.0f);
             slot_zkey[slot] = zk;
+            double w[3];
+            wrapped(d, X[a], Y[a], Z[a], w);
+            slot_wpos[slot] = make_float4((float)w[0], (float)w[1], (float)w[2], 1.0f);
         } else {
             atom_of_slot[slot] = -1;
+            const float nan = __int_as_float(0x7fc00000);     // compares false in every distance test
+            slot_wpos[slot] = make_float4(nan, nan, nan, 0.f);
         }
     }
     // cluster bounding boxes in wrapped coordinates
@@ -243,139 +279,244 @@ __device__ __forceinline__ float bbox_gap2(float4 ca, float4 ha, float4 cb, floa
     return dx * dx + dy * dy + dz * dz;
 }
 
-// One wavefront per super-cluster item.
-__global__ __launch_bounds__(64) void nl_pairlist(const SysDev* sd, SysCount* sc, const int* super_sys,
-                                                  int4* super_item, const int* col_count, const int* col_slot,
-                                                  const float4* cl_ctr, const float4* cl_half,
-                                                  const int* atom_of_slot, const int* slot_of_atom,
-                                                  const int* excl_off, const int* excl_idx, uint2* list,
-                                                  unsigned long long* emask, int4* chunks, int n_items_total) {
+// Position in increasing index order of the k-th column of a periodic window of
+// nwin columns centred on c0 (nwin == nc: every column).  Enumerating windows
+// this way makes the cluster indices of the candidates increase, so the list
+// comes out sorted without a sort.
+__device__ __forceinline__ int window_col(int k, int c0, int D, int nc, int nwin) {
+    if (nwin == nc) return k;
+    const int st = ((c0 - D) % nc + nc) % nc;
+    const int wrapped_front = st + nwin - nc;          // columns past nc - 1, wrapped to 0..
+    if (wrapped_front > 0) return k < wrapped_front ? k : st + (k - wrapped_front);
+    return st + k;
+}
+
+constexpr int kPlThreads = 256;       // threads per super-cluster in nl_pairlist
+
+// Exclusive prefix sum over the kPlThreads threads of a block (thread order);
+// `total` receives the block sum.  Every thread must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kPlThreads / 64; ++w) {
+        const int x = wsum[w];
+        off += (w < wv) ? x : 0;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return off + incl - v;
+}
+
+// One block of kPlThreads per super-cluster item:
+//   1. the columns of the xy window whose rectangle comes within rlist of the
+//      super-cluster's box, in increasing column order;
+//   2. their clusters as one flat candidate range, one thread per candidate:
+//      cluster box against each i-cluster box, ordered compaction;
+//   3. per entry and i-cluster, the 64 atom pairs at build positions, one
+//      wavefront per entry: a tile is kept only if one of its pairs is within
+//      rlist (both atoms move at most skin/2 before the next build, so every
+//      pair that can come within rc is in a kept tile); ordered compaction;
+//   4. exclusion masks for the kept tiles.
+__global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysCount* sc, const int* super_sys,
+                                                          int4* super_item, const int* col_count,
+                                                          const int* col_slot, const float4* cl_ctr,
+                                                          const float4* cl_half, const float4* slot_wpos,
+                                                          const int* atom_of_slot, const int* slot_of_atom,
+                                                          const int* excl_off, const int* excl_idx, uint2* list,
+                                                          unsigned long long* emask, int4* chunks,
+                                                          int n_items_total) {
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= n_items_total) return;
     const int s = super_sys[item];
-    if (!sc[s].rebuild || sc[s].overflow) return;
     const SysDev& d = sd[s];
-    if (item - d.super_off >= sc[s].n_super) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ int go;
+    // one read of the flags per block: another block may raise overflow meanwhile
+    if (tid == 0) go = sc[s].rebuild && !sc[s].overflow && item - d.super_off < sc[s].n_super;
+    __syncthreads();
+    if (!go) return;
     int4 it = super_item[item];
     const int fc = it.x, ncl = it.y;
 
     __shared__ unsigned int l_ent[kListCap];      // cj | imask << 24
     __shared__ int l_eid[kListCap];
-    __shared__ int n_ent;
-    __shared__ unsigned long long x_bits[kExclCap];
-    __shared__ int x_entk[kExclCap];
-    __shared__ int n_x;
+    __shared__ unsigned int x_rec[kExclCap];      // (entry * 8 + i-cluster) << 6 | pair bit
+    __shared__ int n_x, id_base, lbase, cbase;
     __shared__ float4 ic[8], ih[8];
-    if (lane == 0) { n_ent = 0; n_x = 0; }
-    if (lane < 8) {
-        if (lane < ncl) { ic[lane] = cl_ctr[fc + lane]; ih[lane] = cl_half[fc + lane]; }
-        else { ic[lane] = make_float4(1e30f, 1e30f, 1e30f, 0.f); ih[lane] = make_float4(0.f, 0.f, 0.f, 0.f); }
+    __shared__ float4 ip[64];                     // i-atom build positions (NaN for padding)
+    __shared__ int w_first[kPlThreads], w_pre[kPlThreads];
+    __shared__ int wsum[kPlThreads / 64];
+    if (tid == 0) n_x = 0;
+    if (tid < 8) {
+        if (tid < ncl) { ic[tid] = cl_ctr[fc + tid]; ih[tid] = cl_half[fc + tid]; }
+        else { ic[tid] = make_float4(1e30f, 1e30f, 1e30f, 0.f); ih[tid] = make_float4(0.f, 0.f, 0.f, 0.f); }
+    }
+    if (tid < 64) {
+        const float nan = __int_as_float(0x7fc00000);
+        ip[tid] = ((tid >> 3) < ncl) ? slot_wpos[fc * 8 + tid] : make_float4(nan, nan, nan, 0.f);
     }
     __syncthreads();
     const float Lx = (float)d.L[0], Ly = (float)d.L[1], Lz = (float)d.L[2];
+    const float iLx = (float)d.invL[0], iLy = (float)d.invL[1], iLz = (float)d.invL[2];
     const float rl2 = (float)(d.rlist * d.rlist) * 1.0001f;
-    // own column from the first cluster's slot
-    const int own_slot = fc * 8;
-    int cx0 = 0, cy0 = 0;
-    {
-        // locate the column through the cluster centre
-        float4 c0 = ic[0];
-        cx0 = (int)(c0.x / (float)d.colw[0]);
-        cy0 = (int)(c0.y / (float)d.colw[1]);
-        cx0 = min(max(cx0, 0), d.ncx - 1);
-        cy0 = min(max(cy0, 0), d.ncy - 1);
+
+    // xy box of the super-cluster
+    float xlo = 1e30f, xhi = -1e30f, ylo = 1e30f, yhi = -1e30f;
+    for (int k = 0; k < ncl; ++k) {
+        xlo = fminf(xlo, ic[k].x - ih[k].x); xhi = fmaxf(xhi, ic[k].x + ih[k].x);
+        ylo = fminf(ylo, ic[k].y - ih[k].y); yhi = fmaxf(yhi, ic[k].y + ih[k].y);
     }
-    (void)own_slot;
+    const float scx = 0.5f * (xlo + xhi), scy = 0.5f * (ylo + yhi);
+    const float shx = 0.5f * (xhi - xlo), shy = 0.5f * (yhi - ylo);
+    const float cwx = (float)d.colw[0], cwy = (float)d.colw[1];
+
+    // window of columns around the super-cluster's own column
+    int cx0 = (int)(ic[0].x / cwx), cy0 = (int)(ic[0].y / cwy);
+    cx0 = min(max(cx0, 0), d.ncx - 1);
+    cy0 = min(max(cy0, 0), d.ncy - 1);
     const int Dx = (int)ceil((d.rlist + 1e-4) / d.colw[0]) + 1;
     const int Dy = (int)ceil((d.rlist + 1e-4) / d.colw[1]) + 1;
     const int nx_c = (2 * Dx + 1 >= d.ncx) ? d.ncx : 2 * Dx + 1;
     const int ny_c = (2 * Dy + 1 >= d.ncy) ? d.ncy : 2 * Dy + 1;
-    const int sx = (2 * Dx + 1 >= d.ncx) ? 0 : cx0 - Dx;
-    const int sy = (2 * Dy + 1 >= d.ncy) ? 0 : cy0 - Dy;
-    bool overflow = false;
-    for (int ix = 0; ix < nx_c; ++ix) {
-        int cx = sx + ix;
-        cx = ((cx % d.ncx) + d.ncx) % d.ncx;
-        for (int iy = 0; iy < ny_c; ++iy) {
-            int cy = sy + iy;
-            cy = ((cy % d.ncy) + d.ncy) % d.ncy;
-            const int col = d.col_off + cx * d.ncy + cy;
-            const int cnt = col_count[col];
-            const int ccl = (cnt + 7) >> 3;
-            const int cfirst = col_slot[col] >> 3;
-            for (int b = 0; b < ccl; b += 64) {
-                int c = b + lane;
-                unsigned int im = 0;
-                int cj = cfirst + c;
-                if (c < ccl) {
-                    float4 jc = cl_ctr[cj], jh = cl_half[cj];
+    const int nwin = nx_c * ny_c;
+
+    // 1-2: candidates by cluster boxes
+    int ne = 0;                                   // entries so far (the same in every thread)
+    for (int w0 = 0; w0 < nwin; w0 += kPlThreads) {
+        const int w = w0 + tid;
+        int ccl = 0, cfirst = 0;
+        if (w < nwin) {
+            const int cx = window_col(w / ny_c, cx0, Dx, d.ncx, nx_c);
+            const int cy = window_col(w % ny_c, cy0, Dy, d.ncy, ny_c);
+            // gap between the column's rectangle (with 1e-3 nm of slack for binning
+            // round-off) and the super-cluster's xy box, nearest image
+            float gx = fabsf(((float)cx + 0.5f) * cwx - scx), gy = fabsf(((float)cy + 0.5f) * cwy - scy);
+            gx = fmaxf(0.f, fminf(gx, Lx - gx) - 0.5f * cwx - 1e-3f - shx);
+            gy = fmaxf(0.f, fminf(gy, Ly - gy) - 0.5f * cwy - 1e-3f - shy);
+            if (gx * gx + gy * gy < rl2) {
+                const int col = d.col_off + cx * d.ncy + cy;
+                ccl = (col_count[col] + 7) >> 3;
+                cfirst = col_slot[col] >> 3;
+            }
+        }
+        int total;
+        const int pre = block_excl_scan(ccl, wsum, total);
+        w_pre[tid] = pre;
+        w_first[tid] = cfirst;
+        __syncthreads();
+        for (int b = 0; b < total; b += kPlThreads) {
+            const int t = b + tid;
+            unsigned int im = 0;
+            int cj = 0;
+            if (t < total) {
+                int lo = 0, hi = kPlThreads - 1;      // last column whose range starts at or before t
+                while (lo < hi) {
+                    const int m = (lo + hi + 1) >> 1;
+                    if (w_pre[m] <= t) lo = m; else hi = m - 1;
+                }
+                cj = w_first[lo] + (t - w_pre[lo]);
+                const float4 jc = cl_ctr[cj], jh = cl_half[cj];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        if (k < ncl && cj >= fc + k && bbox_gap2(ic[k], ih[k], jc, jh, Lx, Ly, Lz) < rl2) im |= 1u << k;
+                for (int k = 0; k < 8; ++k)
+                    if (k < ncl && cj >= fc + k && bbox_gap2(ic[k], ih[k], jc, jh, Lx, Ly, Lz) < rl2) im |= 1u << k;
+            }
+            int cnt;
+            const int pos = ne + block_excl_scan(im != 0, wsum, cnt);
+            if (im && pos < kListCap) l_ent[pos] = (unsigned)cj | (im << 24);
+            ne += cnt;
+        }
+        __syncthreads();
+    }
+    bool overflow = false;
+    if (ne > kListCap) { overflow = true; ne = kListCap; }
+
+    // 3: atom pairs, one wavefront per entry, batches of eight entries per
+    // wavefront so that their j loads overlap; the kept mask is written in place
+    {
+        const int ia = lane >> 3, ja = lane & 7;
+        float4 pi[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pi[k] = ip[k * 8 + ia];
+        for (int e0 = wv * 8; e0 < ne; e0 += kPlThreads / 8) {
+            unsigned int ent[8];
+            float4 pj[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ent[u] = (e0 + u < ne) ? l_ent[e0 + u] : 0u;
+                pj[u] = slot_wpos[(ent[u] & 0xffffffu) * 8 + ja];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int cj = (int)(ent[u] & 0xffffffu);
+                const unsigned int im = ent[u] >> 24;
+                unsigned int keep = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (im & (1u << k)) {
+                        float dx = pi[k].x - pj[u].x, dy = pi[k].y - pj[u].y, dz = pi[k].z - pj[u].z;
+                        dx -= Lx * rintf(dx * iLx);
+                        dy -= Ly * rintf(dy * iLy);
+                        dz -= Lz * rintf(dz * iLz);
+                        // NaN (padding) compares false
+                        bool ok = dx * dx + dy * dy + dz * dz < rl2;
+                        if (cj == fc + k) ok = ok && ja > ia;
+                        if (__ballot(ok)) keep |= 1u << k;
                     }
                 }
-                unsigned long long bal = __ballot(im != 0);
-                int before = __popcll(bal & ((1ull << lane) - 1ull));
-                int tot = __popcll(bal);
-                int base = n_ent;
-                if (im) {
-                    int pos = base + before;
-                    if (pos < kListCap) { l_ent[pos] = (unsigned)cj | (im << 24); l_eid[pos] = -1; }
-                }
-                __syncthreads();
-                if (lane == 0) n_ent = base + tot;
-                __syncthreads();
+                if (lane == 0 && e0 + u < ne) l_ent[e0 + u] = (unsigned)cj | (keep << 24);
             }
         }
     }
-    int ne = n_ent;
-    if (ne > kListCap) { overflow = true; ne = kListCap; }
-    // sort entries by cluster index (bitonic over the next power of two)
-    int P = 1;
-    while (P < ne) P <<= 1;
-    for (int i = ne + lane; i < P; i += 64) { l_ent[i] = 0xffffffffu; l_eid[i] = -1; }
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < P; i += 64) {
-                int l = i ^ j;
-                if (l > i) {
-                    bool up = (i & k) == 0;
-                    unsigned a = l_ent[i] & 0xffffffu, bb = l_ent[l] & 0xffffffu;
-                    if (l_ent[i] == 0xffffffffu) a = 0xffffffffu;
-                    if (l_ent[l] == 0xffffffffu) bb = 0xffffffffu;
-                    if ((a > bb) == up) { unsigned t = l_ent[i]; l_ent[i] = l_ent[l]; l_ent[l] = t; }
-                }
-            }
+    {
+        int nk = 0;
+        for (int b = 0; b < ne; b += kPlThreads) {
+            const int e = b + tid;
+            const unsigned int v = (e < ne) ? l_ent[e] : 0u;
+            const bool kept = (v >> 24) != 0u;
+            int cnt;
+            const int pos = nk + block_excl_scan(kept, wsum, cnt);   // every read precedes every write
+            if (kept) l_ent[pos] = v;
+            nk += cnt;
             __syncthreads();
         }
+        ne = nk;
     }
-    // exclusions of the 64 i-atoms
+    for (int e = tid; e < ne; e += kPlThreads) l_eid[e] = -1;
+    __syncthreads();
+
+    // 4: exclusions of the 64 i-atoms that fall in kept tiles, four threads per atom
     {
-        const int k = lane >> 3, ia = lane & 7;
+        const int il = tid & 63, part = tid >> 6;
+        const int k = il >> 3, iat = il & 7;
         if (k < ncl) {
-            const int si = (fc + k) * 8 + ia;
+            const int si = (fc + k) * 8 + iat;
             const int ai = atom_of_slot[si];
             if (ai >= 0) {
-                for (int e = excl_off[ai]; e < excl_off[ai + 1]; ++e) {
+                for (int e = excl_off[ai] + part; e < excl_off[ai + 1]; e += kPlThreads / 64) {
                     const int p = excl_idx[e];
                     const int sp = slot_of_atom[p];
                     const int cj = sp >> 3;
                     if (cj < fc + k) continue;
                     if (cj == fc + k && sp <= si) continue;
-                    // binary search cj among entries
-                    int lo = 0, hi = ne;
+                    int lo = 0, hi = ne;              // entries are sorted by cluster
                     while (lo < hi) {
-                        int m = (lo + hi) >> 1;
+                        const int m = (lo + hi) >> 1;
                         if ((int)(l_ent[m] & 0xffffffu) < cj) lo = m + 1; else hi = m;
                     }
                     if (lo < ne && (int)(l_ent[lo] & 0xffffffu) == cj && ((l_ent[lo] >> 24) >> k & 1u)) {
-                        int t = atomicAdd(&n_x, 1);
-                        if (t < kExclCap) {
-                            x_bits[t] = 1ull << (ia * 8 + (sp & 7));
-                            x_entk[t] = lo * 8 + k;
-                        }
+                        const int t = atomicAdd(&n_x, 1);
+                        if (t < kExclCap) x_rec[t] = ((unsigned)(lo * 8 + k) << 6) | (unsigned)(iat * 8 + (sp & 7));
                         l_eid[lo] = 0;      // mark
                     }
                 }
@@ -385,57 +526,63 @@ __global__ __launch_bounds__(64) void nl_pairlist(const SysDev* sd, SysCount* sc
     __syncthreads();
     int nx_ = n_x;
     if (nx_ > kExclCap) { overflow = true; nx_ = kExclCap; }
-    // compact ids for entries with exclusions
-    __shared__ int id_base;
-    if (lane == 0) {
+    // mask ids for the entries with exclusions, in entry order
+    {
         int nm = 0;
-        for (int e = 0; e < ne; ++e) if (l_eid[e] == 0) l_eid[e] = nm++;
-        int b = atomicAdd(&sc[s].emask_count, nm);
-        if (b + nm > d.emask_cap) { id_base = -1; }
-        else id_base = b;
+        for (int b = 0; b < ne; b += kPlThreads) {
+            const int e = b + tid;
+            const bool has = e < ne && l_eid[e] == 0;
+            int cnt;
+            const int r = block_excl_scan(has, wsum, cnt);
+            if (has) l_eid[e] = nm + r;
+            nm += cnt;
+        }
+        if (tid == 0) {
+            const int b = atomicAdd(&sc[s].emask_count, nm);
+            id_base = (b + nm > d.emask_cap) ? -1 : b;
+        }
     }
     __syncthreads();
     if (id_base < 0) overflow = true;
     if (!overflow) {
-        for (int e = lane; e < ne; e += 64) {
+        for (int e = tid; e < ne; e += kPlThreads) {
             if (l_eid[e] >= 0) {
-                int g = d.emask_off + id_base + l_eid[e];
+                const int g = d.emask_off + id_base + l_eid[e];
                 for (int k = 0; k < 8; ++k) emask[(size_t)g * 8 + k] = 0ull;
             }
         }
         __syncthreads();
-        for (int t = lane; t < nx_; t += 64) {
-            int e = x_entk[t] >> 3, k = x_entk[t] & 7;
-            int g = d.emask_off + id_base + l_eid[e];
-            atomicOr(&emask[(size_t)g * 8 + k], x_bits[t]);
+        for (int t = tid; t < nx_; t += kPlThreads) {
+            const int ek = (int)(x_rec[t] >> 6), e = ek >> 3, k = ek & 7;
+            const int g = d.emask_off + id_base + l_eid[e];
+            atomicOr(&emask[(size_t)g * 8 + k], 1ull << (x_rec[t] & 63u));
         }
     }
-    __shared__ int lbase, cbase;
     const int nch = (ne + kChunkEntries - 1) / kChunkEntries;
-    if (lane == 0) {
-        int b = atomicAdd(&sc[s].list_count, ne);
+    if (tid == 0) {
+        const int b = atomicAdd(&sc[s].list_count, ne);
         lbase = (b + ne > d.list_cap) ? -1 : b;
-        int c = atomicAdd(&sc[s].n_chunks, nch);
+        const int c = atomicAdd(&sc[s].n_chunks, nch);
         cbase = (c + nch > d.chunk_cap) ? -1 : c;
     }
     __syncthreads();
     if (lbase < 0 || cbase < 0) overflow = true;
     if (overflow) {
-        if (lane == 0) atomicOr(&sc[s].overflow, 4);
+        if (tid == 0) atomicOr(&sc[s].overflow, 4);
         return;
     }
-    for (int e = lane; e < ne; e += 64) {
+    for (int e = tid; e < ne; e += kPlThreads) {
         uint2 v;
         v.x = l_ent[e];
         v.y = (l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : 0xffffffffu;
         list[d.list_off + lbase + e] = v;
     }
-    if (lane == 0) {
+    if (tid == 0) {
         it.z = d.list_off + lbase;
         it.w = ne;
         super_item[item] = it;
     }
-    for (int c = lane; c < nch; c += 64) {
+    for (int c = tid; c < nch; c += kPlThreads) {
         const int e0 = d.list_off + lbase + c * kChunkEntries;
         const int e1 = d.list_off + lbase + min(ne, (c + 1) * kChunkEntries);
         chunks[d.chunk_off + cbase + c] = make_int4(fc, ncl, e0, e1);
@@ -496,9 +643,9 @@ void launch_nlist(Batch<P>& B) {
                                               B.slot_tmp, B.n_atoms);
     nl_columns<<<B.n_cols, 256, 0, st>>>(B.d_sys, B.d_cnt, B.col_sys, B.col_count, B.col_slot, B.col_super,
                                          B.slot_tmp, B.X, B.Y, B.Z, B.atom_of_slot, B.slot_of_atom, B.cl_ctr,
-                                         B.cl_half, B.super_item, B.slot_zkey);
-    nl_pairlist<<<B.n_super_cap, 64, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
-                                              B.cl_ctr, B.cl_half, B.atom_of_slot, B.slot_of_atom, B.excl_off,
+                                         B.cl_half, B.super_item, B.slot_zkey, B.slot_wpos);
+    nl_pairlist<<<B.n_super_cap, kPlThreads, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
+                                              B.cl_ctr, B.cl_half, B.slot_wpos, B.atom_of_slot, B.slot_of_atom, B.excl_off,
                                               B.excl_idx, B.list, B.emask, B.chunks, B.n_super_cap);
     nl_finish_slots<P><<<nb(B.n_slots_cap, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.sig,
                                                           B.eps, B.lj, B.n_slots_cap);

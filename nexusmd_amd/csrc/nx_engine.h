@@ -50,6 +50,7 @@ struct Batch {
     // columns / clusters / pair list
     int *col_sys = nullptr, *col_count = nullptr, *col_fill = nullptr, *col_slot = nullptr, *col_super = nullptr;
     float4 *cl_ctr = nullptr, *cl_half = nullptr;
+    float4* slot_wpos = nullptr;    // build-time wrapped position of each slot, w = 1 (atom) or 0 (padding)
     int* super_sys = nullptr;
     int4* super_item = nullptr;
     int* chunk_sys = nullptr;

@@ -709,7 +709,7 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
             tw[o + t].y = (real)sin(ang);
         }
     const int maxK = std::max(nx, std::max(ny, nz));
-    const int T = std::max(1, std::min(16, (int)((61440 / (int)sizeof(R2) - maxK) / (2 * maxK))));
+    const int T = std::max(1, std::min(8, (int)((61440 / (int)sizeof(R2) - maxK) / (2 * maxK))));
     std::vector<int4> zi, yi, xi;
     for (int l = 0; l < nx * ny; l += T) zi.push_back(make_int4(0, l, std::min(T, nx * ny - l), 0));
     for (int x = 0; x < nx; ++x)
