@@ -133,16 +133,19 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
 
 /* Kernel timing of the last nx_step/nx_compute_forces call, in milliseconds
  * per launch (average over launches in the call), from HIP events on the
- * context's stream.  Modes:
- *   NX_TIMING_KERNELS     every kernel family launched directly with events
- *                         around it: "nlist", "pack", "bonded", "exclusion",
- *                         "pair", "pme_spread", "fft", "pme_gather", "integrate"
- *   NX_TIMING_GRAPH_PAIR  nx_step replays each step as three graphs (before
- *                         the pair kernel, the pair kernel, after it) with
- *                         events between them: "pair" and "step"
- * Setting NEXUSMD_GRAPHS=0 in the environment before nx_create makes every
- * step run as direct launches (NX_TIMING_GRAPH_PAIR then times as
- * NX_TIMING_KERNELS).
+ * stream each kernel runs on.  A step runs the bonded and exclusion terms and
+ * PME on a second stream beside the pair kernel.  Modes:
+ *   NX_TIMING_KERNELS     every kernel family launched directly, one after
+ *                         the other on one stream, with events around it:
+ *                         "nlist", "pack", "bonded", "atom_forces", "pair",
+ *                         "pme_spread", "fft", "pme_gather", "integrate"
+ *   NX_TIMING_GRAPH_PAIR  nx_step replays each step as graphs around the pair
+ *                         kernel (the terms and PME beside it, the update and
+ *                         the next list after it) with events on the main
+ *                         stream around the pair kernel: "pair" and "step"
+ * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
+ * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);
+ * NEXUSMD_OVERLAP=0 keeps every kernel on one stream.
  * No counterpart in the reference (it has no GPU path to time). */
 enum { NX_TIMING_OFF = 0, NX_TIMING_KERNELS = 1, NX_TIMING_GRAPH_PAIR = 2 };
 nx_status nx_enable_timing(nx_ctx* ctx, int mode);

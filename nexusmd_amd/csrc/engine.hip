@@ -149,11 +149,20 @@ class Engine : public EngineBase {
         // NEXUSMD_GRAPHS=0: every step as direct launches (no graph capture or replay)
         const char* g = std::getenv("NEXUSMD_GRAPHS");
         if (g && std::string(g) == "0") graphs_enabled_ = false;
+        // PME runs on a second stream beside the pair kernel; NEXUSMD_OVERLAP=0 serialises them
+        const char* o = std::getenv("NEXUSMD_OVERLAP");
+        if (o && std::string(o) == "0") overlap_ = false;
+        HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
     }
     ~Engine() override {
         hipSetDevice(device_);
         release();
         if (B.stream) hipStreamDestroy(B.stream);
+        if (side_) hipStreamDestroy(side_);
+        if (fork_) hipEventDestroy(fork_);
+        if (join_) hipEventDestroy(join_);
         for (auto& e : evpool_) hipEventDestroy(e);
     }
 
@@ -176,6 +185,7 @@ class Engine : public EngineBase {
         hs_ = systems;
         const int ns = (int)systems.size();
         B.nsys = ns;
+        HIPCHK(hipDeviceGetAttribute(&B.n_cu, hipDeviceAttributeMultiprocessorCount, device_));
         B.sys.assign(ns, SysDev{});
         long long atom_off = 0, slot_off = 0, col_off = 0, sup_off = 0, list_off = 0, emask_off = 0, chunk_off = 0;
         long long rg_off = 0, cg_off = 0, tw_off = 0, excl_off_total = 0;
@@ -343,7 +353,8 @@ class Engine : public EngineBase {
         dalloc(&B.FSx, S, o);
         dalloc(&B.FSy, S, o);
         dalloc(&B.FSz, S, o);
-        for (int** p : {&B.col_sys, &B.col_count, &B.col_fill, &B.col_slot, &B.col_super}) dalloc(p, B.n_cols, o);
+        for (int** p : {&B.col_sys, &B.col_count, &B.col_fill, &B.col_slot, &B.col_super, &B.col_tmp})
+            dalloc(p, B.n_cols, o);
         dalloc(&B.cl_ctr, S / 8 + 1, o);
         dalloc(&B.cl_half, S / 8 + 1, o);
         dalloc(&B.super_sys, B.n_super_cap, o);
@@ -387,6 +398,21 @@ class Engine : public EngineBase {
         B.fft_lines = T;
         B.fft_tile = T;
         B.fft_maxK = maxK;
+        // plane passes when every system's planes fit one workgroup's LDS; the x
+        // pass splits the kz range so that the batch has about 512 x blocks
+        bool plane = fft_plane_enabled();
+        int sum_ky = 0;
+        for (int s = 0; s < ns; ++s) {
+            const SysDev& d = B.sys[s];
+            if (plane_lds_bytes<real2>(d.grid[1], d.grid[2]) > kMaxBlockLds ||
+                xpass_lds_bytes<real2>(d.grid[0], 1) > kMaxBlockLds)
+                plane = false;
+            sum_ky += d.grid[1];
+        }
+        const int xsplit = std::max(1, (512 + sum_ky - 1) / std::max(1, sum_ky));
+        B.fft_plane = plane ? 1 : 0;
+        B.fft_plane_lds = 0;
+        B.fft_x_lds = 0;
         for (int s = 0; s < ns; ++s) {
             HostSystem& h = *hs_[s];
             SysDev& d = B.sys[s];
@@ -466,11 +492,21 @@ class Engine : public EngineBase {
             } else {
                 direct_slots += d.slot_cap;
             }
-            for (int l = 0; l < Kx * Ky; l += T) zi.push_back(make_int4(s, l, std::min(T, Kx * Ky - l), 0));
-            for (int x = 0; x < Kx; ++x)
-                for (int k = 0; k < Kzh; k += T) yi.push_back(make_int4(s, x, k, 0));
-            for (int y = 0; y < Ky; ++y)
-                for (int k = 0; k < Kzh; k += T) xi.push_back(make_int4(s, y, k, 0));
+            if (B.fft_plane) {
+                for (int x = 0; x < Kx; ++x) zi.push_back(make_int4(s, x, 0, 0));
+                int nk = (Kzh + xsplit - 1) / xsplit;
+                while (nk > 1 && xpass_lds_bytes<real2>(Kx, nk) > kMaxBlockLds) nk = (nk + 1) / 2;
+                for (int y = 0; y < Ky; ++y)
+                    for (int k = 0; k < Kzh; k += nk) xi.push_back(make_int4(s, y, k, std::min(nk, Kzh - k)));
+                B.fft_plane_lds = std::max(B.fft_plane_lds, plane_lds_bytes<real2>(Ky, Kz));
+                B.fft_x_lds = std::max(B.fft_x_lds, xpass_lds_bytes<real2>(Kx, nk));
+            } else {
+                for (int l = 0; l < Kx * Ky; l += T) zi.push_back(make_int4(s, l, std::min(T, Kx * Ky - l), 0));
+                for (int x = 0; x < Kx; ++x)
+                    for (int k = 0; k < Kzh; k += T) yi.push_back(make_int4(s, x, k, 0));
+                for (int y = 0; y < Ky; ++y)
+                    for (int k = 0; k < Kzh; k += T) xi.push_back(make_int4(s, y, k, 0));
+            }
             self_energy_.resize(ns);
             double se = 0.0;
             for (int i = 0; i < h.n; ++i) se -= kOne4PiEps0 * h.q[i] * h.q[i] * d.alpha / kSqrtPi;
@@ -517,6 +553,7 @@ class Engine : public EngineBase {
         B.n_fftz = (int)zi.size();
         B.n_ffty = (int)yi.size();
         B.n_fftx = (int)xi.size();
+        HIPCHK(prepare_fft<P>(B));
         upload(B.d_sys, B.sys, st);
         upload(B.mass, mass, st);
         upload(B.invm, invm, st);
@@ -570,8 +607,10 @@ class Engine : public EngineBase {
         request_rebuild();
         HIPCHK(hipStreamSynchronize(st));
         // first build; re-lay out with larger list capacities until the lists fit
-        launch_nlist<P>(B);
+        launch_nlist<P>(B, true);
+        // positions moved on from the build positions: repack before the next forces
         if (carried) { upload(B.X, x, st); upload(B.Y, y, st); upload(B.Z, z, st); }
+        pack_needed_ = carried;
         HIPCHK(hipStreamSynchronize(st));
         std::vector<SysCount> c(ns);
         copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * ns, hipMemcpyDeviceToHost);
@@ -642,28 +681,92 @@ class Engine : public EngineBase {
         ev_used_ = 0;
     }
 
-    // One force evaluation in three parts: 0 = list, packing, bonded and
-    // exclusion terms; 1 = the pair kernel; 2 = PME.  -1 = all of them.
-    void forces_part(int mask, bool energy, int part) {
-        if (part <= 0) {
-            timed("nlist", [&] { launch_nlist<P>(B); });
-            timed("pack", [&] { launch_pack<P>(B); });
+    void pme(bool energy) {
+        timed("pme_spread", [&] { launch_pme_spread<P>(B); });
+        timed("fft", [&] {
+            launch_fft_forward<P>(B);
+            launch_fft_convolve<P>(B, energy);
+            launch_fft_inverse<P>(B);
+        });
+        timed("pme_gather", [&] { launch_pme_gather<P>(B); });
+    }
+
+    // Launches f on the side stream: the kernels it launches read B.stream.
+    template <class F>
+    void on_side(F&& f) {
+        struct Restore {
+            hipStream_t& ref;
+            hipStream_t saved;
+            ~Restore() { ref = saved; }
+        } restore{B.stream, B.stream};
+        B.stream = side_;
+        f();
+    }
+
+    // Parts of one force evaluation: list and slot positions; bonded and
+    // exclusion terms; the pair kernel; PME.
+    enum { PART_LIST = 1, PART_TERMS = 2, PART_PAIR = 4, PART_PME = 8, PART_ALL = 15 };
+
+    // Whether the terms and PME run on the side stream beside the pair kernel
+    // (not under per-kernel timing, whose times are those of kernels alone).
+    bool use_side(int mask) const {
+        return overlap_ && (mask & NX_FORCE_NONBONDED) && B.pme_any && B.timing != NX_TIMING_KERNELS;
+    }
+
+    // The bonded and exclusion terms and PME (the work beside the pair kernel).
+    void side_work(int mask, bool energy) {
+        if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
+        timed("atom_forces", [&] { launch_atom_forces<P>(B, mask, energy); });
+        if (mask & NX_FORCE_NONBONDED) pme(energy);
+    }
+
+    // The parts in `parts`.  With every part and the side stream in use, the
+    // terms and PME go to the side stream after the slot positions and run
+    // beside the pair kernel (the pair kernel and PME add to the slot forces
+    // with atomics, the terms write the atom forces); the main stream then
+    // waits for the side stream.
+    void forces_part(int mask, bool energy, int parts) {
+        const bool nb = (mask & NX_FORCE_NONBONDED) != 0;
+        if (parts & PART_LIST) {
+            // the prefix kernel of the pair kernel clears the rebuild flags
+            timed("nlist", [&] { launch_nlist<P>(B, !nb); });
+            settle();
+        }
+        if (parts == PART_ALL && use_side(mask)) {
+            fork_side();
+            on_side([&] { side_work(mask, energy); });
+            timed("pair", [&] { launch_pair<P>(B, energy); });
+            join_side();
+            return;
+        }
+        if (parts & PART_TERMS) {
             if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
             timed("atom_forces", [&] { launch_atom_forces<P>(B, mask, energy); });
         }
-        if (!(mask & NX_FORCE_NONBONDED)) return;
-        if (part < 0 || part == 1) timed("pair", [&] { launch_pair<P>(B, energy); });
-        if (part < 0 || part == 2) {
-            timed("pme_spread", [&] { launch_pme_spread<P>(B); });
-            timed("fft", [&] {
-                launch_fft_forward<P>(B);
-                launch_fft_convolve<P>(B, energy);
-                launch_fft_inverse<P>(B);
-            });
-            timed("pme_gather", [&] { launch_pme_gather<P>(B); });
-        }
+        if (nb && (parts & PART_PAIR)) timed("pair", [&] { launch_pair<P>(B, energy); });
+        if (nb && (parts & PART_PME)) pme(energy);
     }
-    void forces(int mask, bool energy) { forces_part(mask, energy, -1); }
+    void forces(int mask, bool energy) { forces_part(mask, energy, PART_ALL); }
+
+    // Slot positions and slot forces are kept current by the list build and the
+    // update; after positions were replaced or forces computed without an update,
+    // one packing pass restores them (outside any graph).
+    void settle() {
+        if (!pack_needed_ || capturing_) return;
+        timed("pack", [&] { launch_pack<P>(B); });
+        pack_needed_ = false;
+    }
+
+    // The side stream waits for the work queued so far on the main stream; the
+    // main stream waits for the side stream.
+    void fork_side() {
+        HIPCHK(hipEventRecord(fork_, B.stream));
+        HIPCHK(hipStreamWaitEvent(side_, fork_, 0));
+    }
+    void join_side() {
+        HIPCHK(hipEventRecord(join_, side_));
+        HIPCHK(hipStreamWaitEvent(B.stream, join_, 0));
+    }
 
     void zero_energy() { HIPCHK(hipMemsetAsync(B.energy, 0, sizeof(long long) * B.nsys * E_NTERMS, B.stream)); }
 
@@ -702,19 +805,30 @@ class Engine : public EngineBase {
 
     void run(int nsteps, double dt, int mask) override {
         begin_call();
+        settle();
         if (B.timing == NX_TIMING_GRAPH_PAIR && nsteps > 0 && graphs_enabled_) {
-            // the step as three graphs (before the pair kernel, the pair kernel,
-            // after it) with events on the stream between the replays
+            // graphs around the pair kernel, with events on the main stream around
+            // it: [list, positions] once, then per step the side graph (terms,
+            // PME; on the side stream beside the pair kernel), the pair kernel and
+            // [update, list, positions] ([update] after the last step)
             if (!seg_exec_[0] || graph_dt_seg_ != dt || graph_mask_seg_ != mask) build_segments(dt, mask);
+            const bool side = use_side(mask);
             hipEvent_t e0 = ev();
             HIPCHK(hipEventRecord(e0, B.stream));
+            HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
             for (int i = 0; i < nsteps; ++i) {
                 hipEvent_t a = ev(), b = ev(), c = ev();
-                HIPCHK(hipGraphLaunch(seg_exec_[0], B.stream));
+                if (side) {
+                    fork_side();
+                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], side_));
+                } else {
+                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
+                }
                 HIPCHK(hipEventRecord(a, B.stream));
-                HIPCHK(hipGraphLaunch(seg_exec_[1], B.stream));
+                if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
                 HIPCHK(hipEventRecord(b, B.stream));
-                HIPCHK(hipGraphLaunch(seg_exec_[2], B.stream));
+                if (side) join_side();
+                HIPCHK(hipGraphLaunch(seg_exec_[i + 1 < nsteps ? SEG_MID : SEG_LAST], B.stream));
                 HIPCHK(hipEventRecord(c, B.stream));
                 evs_.push_back({"pair", a, b});
                 evs_.push_back({"step", e0, c});
@@ -755,11 +869,13 @@ class Engine : public EngineBase {
         const int saved = B.timing;
         B.timing = 0;
         capturing_ = true;
-        for (int g = 0; g < 3; ++g) {
+        for (int g = 0; g < SEG_COUNT; ++g) {
+            if (g == SEG_PAIR && !(mask & NX_FORCE_NONBONDED)) continue;     // no pair graph
             HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-            if (g == 0) forces_part(mask, false, 0);
-            if (g == 1) forces_part(mask, false, 1);
-            if (g == 2) { forces_part(mask, false, 2); launch_integrate<P>(B, dt); }
+            if (g == SEG_MID || g == SEG_LAST) launch_integrate<P>(B, dt);
+            if (g == SEG_FIRST || g == SEG_MID) forces_part(mask, false, PART_LIST);
+            if (g == SEG_SIDE) side_work(mask, false);
+            if (g == SEG_PAIR) launch_pair<P>(B, false);
             hipError_t e = hipStreamEndCapture(B.stream, &seg_[g]);
             if (e != hipSuccess) { capturing_ = false; B.timing = saved; HIPCHK(e); }
             HIPCHK(hipGraphInstantiate(&seg_exec_[g], seg_[g], nullptr, nullptr, 0));
@@ -770,7 +886,7 @@ class Engine : public EngineBase {
         graph_mask_seg_ = mask;
     }
     void destroy_segments() {
-        for (int g = 0; g < 3; ++g) {
+        for (int g = 0; g < SEG_COUNT; ++g) {
             if (seg_exec_[g]) { hipGraphExecDestroy(seg_exec_[g]); seg_exec_[g] = nullptr; }
             if (seg_[g]) { hipGraphDestroy(seg_[g]); seg_[g] = nullptr; }
         }
@@ -816,6 +932,8 @@ class Engine : public EngineBase {
         if (integrate) {
             timed("integrate", [&] { launch_integrate<P>(B, dt); });
             ++steps_;
+        } else {
+            pack_needed_ = true;          // the slot forces still hold this evaluation
         }
         end_call();
     }
@@ -875,7 +993,7 @@ class Engine : public EngineBase {
     long long export_pairs(std::vector<int>* out) override {
         begin_call();
         // make the list current for these positions
-        launch_nlist<P>(B);
+        launch_nlist<P>(B, true);
         HIPCHK(hipStreamSynchronize(B.stream));
         check_counters();
         long long cap = out ? (1 << 20) : 0;
@@ -996,8 +1114,15 @@ class Engine : public EngineBase {
     int graph_mask_ = -1;
     int failed_sys_ = -1;
     int grow_depth_ = 0;
-    hipGraph_t seg_[3] = {nullptr, nullptr, nullptr};
-    hipGraphExec_t seg_exec_[3] = {nullptr, nullptr, nullptr};
+    // timing segments: [list, positions], [terms, PME], [pair], [update, list,
+    // positions], [update]
+    enum { SEG_FIRST, SEG_SIDE, SEG_PAIR, SEG_MID, SEG_LAST, SEG_COUNT };
+    hipGraph_t seg_[SEG_COUNT] = {};
+    hipGraphExec_t seg_exec_[SEG_COUNT] = {};
+    hipStream_t side_ = nullptr;            // PME beside the pair kernel
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    bool overlap_ = true;
+    bool pack_needed_ = false;              // slot positions/forces stale (see settle)
     double graph_dt_seg_ = 0.0;
     int graph_mask_seg_ = -1;
     std::vector<SysCount> counts_;

@@ -13,6 +13,8 @@
 namespace nx {
 
 // Slot-order copy of (x, y, z, q) for the pair and PME kernels; clears F_slot.
+// Needed only where positions or slot forces changed outside a step (the
+// list build and the update keep both current otherwise).
 template <class P>
 __global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
                             const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
@@ -85,22 +87,31 @@ __global__ void kinetic_kernel(const int* __restrict__ atom_sys, const int* __re
     wave_energy_add(energy, s < 0 ? 0 : s, E_KIN, ke, s >= 0);
 }
 
+// Leapfrog update; also writes the slot-order position the next step's force
+// kernels read (x, y, z, q at the atom's slot) and clears the slot force it
+// has consumed, so no packing pass runs between builds.
 template <class P>
 __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, const int* __restrict__ atom_sys,
                                  const int* __restrict__ slot_of_atom, double* X, double* Y, double* Z, double* VX,
                                  double* VY, double* VZ, const double* __restrict__ invm,
-                                 const double* __restrict__ XB, const double* __restrict__ YB,
-                                 const double* __restrict__ ZB, typename P::facc* FAx, typename P::facc* FAy,
-                                 typename P::facc* FAz, const typename P::facc* FSx, const typename P::facc* FSy,
-                                 const typename P::facc* FSz, double dt, int n_atoms) {
+                                 const double* __restrict__ Q, const double* __restrict__ XB,
+                                 const double* __restrict__ YB, const double* __restrict__ ZB,
+                                 typename P::facc* FAx, typename P::facc* FAy, typename P::facc* FAz,
+                                 typename P::facc* FSx, typename P::facc* FSy, typename P::facc* FSz,
+                                 typename P::real4* pos, double dt, int n_atoms) {
+    using real = typename P::real;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     int s = 0;
     if (a < n_atoms) {
         s = atom_sys[a];
         const SysDev& d = sd[s];
+        const int sl = slot_of_atom[a];
         double f[3];
-        total_force<P>(a, slot_of_atom[a], FAx, FAy, FAz, FSx, FSy, FSz, f);
+        total_force<P>(a, sl, FAx, FAy, FAz, FSx, FSy, FSz, f);
+        FSx[sl] = 0;
+        FSy[sl] = 0;
+        FSz[sl] = 0;
         const double im = invm[a];
         double x = X[a], y = Y[a], z = Z[a];
         if (im != 0.0) {
@@ -117,6 +128,12 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
             X[a] = x; Y[a] = y; Z[a] = z;
             VX[a] = vx; VY[a] = vy; VZ[a] = vz;
         }
+        typename P::real4 ps;
+        ps.x = (real)x;
+        ps.y = (real)y;
+        ps.z = (real)z;
+        ps.w = (real)Q[a];
+        pos[sl] = ps;
         const double dx = min_image1(x - XB[a], d.L[0]);
         const double dy = min_image1(y - YB[a], d.L[1]);
         const double dz = min_image1(z - ZB[a], d.L[2]);
@@ -160,8 +177,8 @@ template <class P>
 void launch_integrate(Batch<P>& B, double dt) {
     const int T = 256;
     integrate_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
-        B.d_sys, B.d_cnt, B.atom_sys, B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.invm, B.XB, B.YB, B.ZB, B.FAx,
-        B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, dt, B.n_atoms);
+        B.d_sys, B.d_cnt, B.atom_sys, B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.invm, B.q, B.XB, B.YB, B.ZB,
+        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.pos, dt, B.n_atoms);
 }
 
 template <class P>

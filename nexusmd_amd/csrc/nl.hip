@@ -6,11 +6,14 @@
 //   1. nl_bin      : counting sort of atoms into xy columns (one atomic per
 //                    distinct column in a wavefront);
 //   2. nl_scan     : per-system exclusive scan -> slot ranges (columns padded
-//                    to 8-atom clusters) and super-cluster ranges;
+//                    to 8-atom clusters) and super-cluster ranges; clears the
+//                    bin counters for the next build;
 //   3. nl_scatter  : atoms into their column's slot range;
 //   4. nl_columns  : per column, sort by (z, atom id) in LDS -> deterministic
 //                    slot order; cluster bounding boxes; build positions in
-//                    slot order;
+//                    slot order; the slot-order positions, charges and LJ
+//                    parameters the force kernels read, and each atom's build
+//                    position for the displacement test;
 //   5. nl_pairlist : one block per super-cluster (8 clusters of 8 atoms):
 //                    j-clusters of nearby columns whose bounding box is within
 //                    rlist of an i-cluster (half list: j >= i), then per tile
@@ -21,7 +24,10 @@
 // set is the reference's set whatever the buffer (skin) is.
 //
 // Every kernel first checks the system's rebuild flag, which the integrator
-// raises when an atom has moved more than skin/2 since the last build.
+// raises when an atom has moved more than skin/2 since the last build.  The
+// pair kernel's prefix kernel clears it (nl_clear_flag when no pair kernel
+// follows).  Between builds the integrator keeps the slot-order positions
+// current.
 #include "nx_device.h"
 #include "nx_engine.h"
 
@@ -30,16 +36,6 @@ namespace nx {
 constexpr int kColSortCap = 2048;     // atoms per column sorted in LDS
 constexpr int kListCap = 1024;        // pair-list entries per super-cluster (LDS)
 constexpr int kExclCap = 2048;        // exclusion triples per super-cluster (LDS)
-
-__global__ void nl_zero(const SysDev* sd, SysCount* sc, int* col_count, int* col_fill, int n_cols_total,
-                        const int* col_sys) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cols_total) return;
-    int s = col_sys[c];
-    if (!sc[s].rebuild) return;
-    col_count[c] = 0;
-    col_fill[c] = 0;
-}
 
 // Column of an atom from its wrapped position.
 __device__ __forceinline__ void wrapped(const SysDev& d, double x, double y, double z, double* w) {
@@ -80,8 +76,9 @@ __device__ __forceinline__ int wave_keyed_increment(int* counter, int key) {
     return res;
 }
 
+// Column of every atom and the column counts (into col_tmp, zero between builds).
 __global__ void nl_bin(const SysDev* sd, SysCount* sc, const int* atom_sys, const double* X, const double* Y,
-                       const double* Z, int* atom_col, int* col_count, int n_atoms_total) {
+                       const double* Z, int* atom_col, int* col_tmp, int n_atoms_total) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_atoms_total) return;
     int s = atom_sys[a];
@@ -95,11 +92,13 @@ __global__ void nl_bin(const SysDev* sd, SysCount* sc, const int* atom_sys, cons
     cy = cy < 0 ? 0 : (cy >= d.ncy ? d.ncy - 1 : cy);
     int col = d.col_off + cx * d.ncy + cy;
     atom_col[a] = col;
-    wave_keyed_increment(col_count, col);
+    wave_keyed_increment(col_tmp, col);
 }
 
-// One block per system: scan padded column counts.
-__global__ void nl_scan(const SysDev* sd, SysCount* sc, const int* col_count, int* col_slot, int* col_super) {
+// One block per system: scan padded column counts; col_tmp -> col_count and
+// back to zero.
+__global__ void nl_scan(const SysDev* sd, SysCount* sc, int* col_tmp, int* col_count, int* col_slot,
+                        int* col_super) {
     int s = blockIdx.x;
     if (!sc[s].rebuild) return;
     const SysDev& d = sd[s];
@@ -110,7 +109,12 @@ __global__ void nl_scan(const SysDev* sd, SysCount* sc, const int* col_count, in
     __syncthreads();
     for (int base = 0; base < nc; base += blockDim.x) {
         int c = base + threadIdx.x;
-        int cnt = (c < nc) ? col_count[d.col_off + c] : 0;
+        int cnt = 0;
+        if (c < nc) {
+            cnt = col_tmp[d.col_off + c];
+            col_count[d.col_off + c] = cnt;
+            col_tmp[d.col_off + c] = 0;
+        }
         int ps = (cnt + 7) & ~7;
         int pu = (cnt + 63) >> 6;
         ws[threadIdx.x] = ps;
@@ -159,16 +163,24 @@ __global__ void nl_scatter(const SysDev* sd, SysCount* sc, const int* atom_sys, 
     if (slot < sd[s].slot_off + sd[s].slot_cap) slot_atom_tmp[slot] = a;
 }
 
-// One block (256 threads) per column: deterministic z order, cluster boxes, super items.
+// One block (256 threads) per column: deterministic z order, cluster boxes,
+// super items, and the slot-order data of the force kernels (pos: x, y, z, q;
+// NaN coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).
+template <class P>
 __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc, const int* col_sys,
-                                                  const int* col_count, const int* col_slot, const int* col_super,
-                                                  const int* slot_atom_tmp, const double* X, const double* Y,
-                                                  const double* Z, int* atom_of_slot, int* slot_of_atom,
-                                                  float4* cl_ctr, float4* cl_half, int4* super_item,
-                                                  float* slot_zkey, float4* slot_wpos) {
+                                                  const int* col_count, int* col_fill, const int* col_slot,
+                                                  const int* col_super, const int* slot_atom_tmp, const double* X,
+                                                  const double* Y, const double* Z, const double* Q,
+                                                  const double* sig, const double* eps, int* atom_of_slot,
+                                                  int* slot_of_atom, float4* cl_ctr, float4* cl_half,
+                                                  int4* super_item, float* slot_zkey, float4* slot_wpos,
+                                                  typename P::real4* pos, typename P::real2* lj, double* XB,
+                                                  double* YB, double* ZB) {
+    using real = typename P::real;
     const int col = blockIdx.x;
     const int s = col_sys[col];
     if (!sc[s].rebuild || (sc[s].overflow & 1)) return;
+    if (threadIdx.x == 0) col_fill[col] = 0;          // the scatter is done: zero for the next build
     const SysDev& d = sd[s];
     const int cnt = col_count[col];
     const int first = col_slot[col];
@@ -178,9 +190,9 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
         if (threadIdx.x == 0) atomicOr(&sc[s].overflow, 2);
         return;
     }
-    int P = 1;
-    while (P < cnt) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    int np2 = 1;
+    while (np2 < cnt) np2 <<= 1;
+    for (int i = threadIdx.x; i < np2; i += blockDim.x) {
         if (i < cnt) {
             int a = slot_atom_tmp[first + i];
             double w[3];
@@ -194,9 +206,9 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
     }
     __syncthreads();
     // bitonic sort by (z, atom id)
-    for (int k = 2; k <= P; k <<= 1) {
+    for (int k = 2; k <= np2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            for (int i = threadIdx.x; i < np2; i += blockDim.x) {
                 int l = i ^ j;
                 if (l > i) {
                     bool up = (i & k) == 0;
@@ -221,13 +233,35 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
             float zk = (float)key[i];
             if ((double)zk > key[i]) zk = nextafterf(zk, -1.0f);
             slot_zkey[slot] = zk;
+            const double x = X[a], y = Y[a], z = Z[a];
             double w[3];
-            wrapped(d, X[a], Y[a], Z[a], w);
+            wrapped(d, x, y, z, w);
             slot_wpos[slot] = make_float4((float)w[0], (float)w[1], (float)w[2], 1.0f);
+            typename P::real4 p;
+            p.x = (real)x;
+            p.y = (real)y;
+            p.z = (real)z;
+            p.w = (real)Q[a];
+            pos[slot] = p;
+            typename P::real2 l;
+            l.x = (real)(0.5 * sig[a]);
+            l.y = (real)(2.0 * sqrt(eps[a]));
+            lj[slot] = l;
+            XB[a] = x;
+            YB[a] = y;
+            ZB[a] = z;
         } else {
             atom_of_slot[slot] = -1;
             const float nan = __int_as_float(0x7fc00000);     // compares false in every distance test
             slot_wpos[slot] = make_float4(nan, nan, nan, 0.f);
+            typename P::real4 p;
+            p.x = p.y = p.z = (real)__builtin_nan("");
+            p.w = 0;
+            pos[slot] = p;
+            typename P::real2 l;
+            l.x = 0;
+            l.y = 0;
+            lj[slot] = l;
         }
     }
     // cluster bounding boxes in wrapped coordinates
@@ -589,38 +623,6 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     }
 }
 
-// After a build: pack per-slot LJ parameters, record build positions, clear the flag.
-template <class P>
-__global__ void nl_finish_slots(const SysDev* sd, SysCount* sc, const int* slot_sys, const int* atom_of_slot,
-                                const double* sig, const double* eps, typename P::real2* lj, int n_slots_total) {
-    int sl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sl >= n_slots_total) return;
-    int s = slot_sys[sl];
-    if (!sc[s].rebuild || sc[s].overflow) return;
-    int a = (sl - sd[s].slot_off < sc[s].n_slots) ? atom_of_slot[sl] : -1;
-    typename P::real2 v;
-    if (a >= 0) {
-        v.x = (typename P::real)(0.5 * sig[a]);
-        v.y = (typename P::real)(2.0 * sqrt(eps[a]));
-    } else {
-        v.x = 0;
-        v.y = 0;
-    }
-    lj[sl] = v;
-}
-
-__global__ void nl_finish_atoms(const SysDev* sd, SysCount* sc, const int* atom_sys, const double* X,
-                                const double* Y, const double* Z, double* XB, double* YB, double* ZB,
-                                int n_atoms_total) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_atoms_total) return;
-    int s = atom_sys[a];
-    if (!sc[s].rebuild || sc[s].overflow) return;
-    XB[a] = X[a];
-    YB[a] = Y[a];
-    ZB[a] = Z[a];
-}
-
 __global__ void nl_clear_flag(SysCount* sc, int nsys) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
@@ -631,30 +633,26 @@ __global__ void nl_clear_flag(SysCount* sc, int nsys) {
 }
 
 template <class P>
-void launch_nlist(Batch<P>& B) {
+void launch_nlist(Batch<P>& B, bool clear_flag) {
     const int T = 256;
     hipStream_t st = B.stream;
     auto nb = [](long n, int t) { return (unsigned)((n + t - 1) / t); };
-    nl_zero<<<nb(B.n_cols, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.col_count, B.col_fill, B.n_cols, B.col_sys);
-    nl_bin<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.atom_col, B.col_count,
+    nl_bin<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.atom_col, B.col_tmp,
                                           B.n_atoms);
-    nl_scan<<<B.nsys, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.col_count, B.col_slot, B.col_super);
+    nl_scan<<<B.nsys, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.col_tmp, B.col_count, B.col_slot, B.col_super);
     nl_scatter<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.atom_col, B.col_slot, B.col_fill,
                                               B.slot_tmp, B.n_atoms);
-    nl_columns<<<B.n_cols, 256, 0, st>>>(B.d_sys, B.d_cnt, B.col_sys, B.col_count, B.col_slot, B.col_super,
-                                         B.slot_tmp, B.X, B.Y, B.Z, B.atom_of_slot, B.slot_of_atom, B.cl_ctr,
-                                         B.cl_half, B.super_item, B.slot_zkey, B.slot_wpos);
+    nl_columns<P><<<B.n_cols, 256, 0, st>>>(B.d_sys, B.d_cnt, B.col_sys, B.col_count, B.col_fill, B.col_slot,
+                                            B.col_super, B.slot_tmp, B.X, B.Y, B.Z, B.q, B.sig, B.eps,
+                                            B.atom_of_slot, B.slot_of_atom, B.cl_ctr, B.cl_half, B.super_item,
+                                            B.slot_zkey, B.slot_wpos, B.pos, B.lj, B.XB, B.YB, B.ZB);
     nl_pairlist<<<B.n_super_cap, kPlThreads, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
                                               B.cl_ctr, B.cl_half, B.slot_wpos, B.atom_of_slot, B.slot_of_atom, B.excl_off,
                                               B.excl_idx, B.list, B.emask, B.chunks, B.n_super_cap);
-    nl_finish_slots<P><<<nb(B.n_slots_cap, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.sig,
-                                                          B.eps, B.lj, B.n_slots_cap);
-    nl_finish_atoms<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.XB, B.YB, B.ZB,
-                                                   B.n_atoms);
-    nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
+    if (clear_flag) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
 }
 
-template void launch_nlist<PrecFP64>(Batch<PrecFP64>&);
-template void launch_nlist<PrecMixed>(Batch<PrecMixed>&);
+template void launch_nlist<PrecFP64>(Batch<PrecFP64>&, bool);
+template void launch_nlist<PrecMixed>(Batch<PrecMixed>&, bool);
 
 }  // namespace nx

@@ -102,14 +102,19 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Chunk prefix over systems: work item g of the whole batch maps to system s
 // with prefix[s] <= g < prefix[s+1] and chunk sd[s].chunk_off + g - prefix[s].
-__global__ void pair_prefix_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc, int nsys,
-                                   int* prefix) {
+// Also ends the step's list work: clears the rebuild flags (the list kernels
+// that read them ran before) and counts the builds.
+__global__ void pair_prefix_kernel(const SysDev* __restrict__ sd, SysCount* sc, int nsys, int* prefix) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int t = 0;
     for (int s = 0; s < nsys; ++s) {
         prefix[s] = t;
         // the counter runs past the capacity when a build overflows
         t += min(sc[s].n_chunks, sd[s].chunk_cap);
+        if (sc[s].rebuild) {
+            sc[s].rebuild = 0;
+            sc[s].rebuilds += 1;
+        }
     }
     prefix[nsys] = t;
 }
@@ -396,12 +401,28 @@ __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, cons
     }
 }
 
+// Blocks of the pair kernel one CU holds at once (its occupancy limit).
+template <class P, bool ENERGY>
+static int pair_blocks_per_cu() {
+    static const int v = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_kernel<P, ENERGY>, 256, 0) != hipSuccess || n <= 0)
+            n = 2;
+        return n;
+    }();
+    return v;
+}
+
 template <class P>
 void launch_pair(Batch<P>& B, bool energy) {
     if (B.n_chunk_cap == 0) return;
     pair_prefix_kernel<<<1, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_prefix);
-    // persistent grid: ~24 resident wavefronts per CU on a 256-CU part
-    const unsigned blocks = (unsigned)std::max(1, std::min((B.n_chunk_cap + kPairWaves - 1) / kPairWaves, 1536));
+    // persistent grid, all blocks resident at once; where a CU holds four or
+    // more, one block's worth is left to the PME kernels running beside
+    const int occ = energy ? pair_blocks_per_cu<P, true>() : pair_blocks_per_cu<P, false>();
+    const int per_cu = occ >= 4 ? occ - 1 : occ;
+    const unsigned blocks = (unsigned)std::max(
+        1, std::min((B.n_chunk_cap + kPairWaves - 1) / kPairWaves, per_cu * std::max(1, B.n_cu)));
     if (energy)
         pair_kernel<P, true><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.chunks, B.list,
                                                            B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);

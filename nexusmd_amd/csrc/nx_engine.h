@@ -17,6 +17,25 @@ namespace nx {
 
 struct HostSystem;   // engine.hip
 
+// Plane FFT passes: threads per block, LDS available to one workgroup on gfx950.
+constexpr int kPlaneThreads = 1024;
+constexpr int kXPassThreads = 512;
+constexpr size_t kMaxBlockLds = 160 * 1024 - 1024;   // less the kernels' static LDS
+
+// LDS bytes of a plane pass (two buffers, the y and z twiddles).
+template <typename R2>
+inline size_t plane_lds_bytes(int Ky, int Kz) {
+    const int np = (Ky + 1) / 2, Kzh = Kz / 2 + 1;
+    const int cap = np * Kz > Ky * Kzh ? np * Kz : Ky * Kzh;
+    return (size_t)(Ky + Kz + 2 * cap) * sizeof(R2);
+}
+
+// LDS bytes of an x pass over nk wave numbers kz (two buffers, the twiddles).
+template <typename R2>
+inline size_t xpass_lds_bytes(int Kx, int nk) {
+    return (size_t)(Kx + 2 * nk * Kx) * sizeof(R2);
+}
+
 template <class P>
 struct Batch {
     using real = typename P::real;
@@ -49,6 +68,7 @@ struct Batch {
 
     // columns / clusters / pair list
     int *col_sys = nullptr, *col_count = nullptr, *col_fill = nullptr, *col_slot = nullptr, *col_super = nullptr;
+    int* col_tmp = nullptr;         // bin counts during a build (zero between builds)
     float4 *cl_ctr = nullptr, *cl_half = nullptr;
     float4* slot_wpos = nullptr;    // build-time wrapped position of each slot, w = 1 (atom) or 0 (padding)
     int* super_sys = nullptr;
@@ -56,6 +76,7 @@ struct Batch {
     int* chunk_sys = nullptr;
     int4* chunks = nullptr;         // (first cluster, n clusters, first entry, end entry)
     int* chunk_prefix = nullptr;    // [nsys + 1] running total of chunks over systems
+    int n_cu = 0;                   // compute units of the device
     int n_chunk_cap = 0;
     uint2* list = nullptr;
     unsigned long long* emask = nullptr;
@@ -93,6 +114,8 @@ struct Batch {
     int4* fftx_items = nullptr;     // (sys, y, kz0, 0)
     int n_fftz = 0, n_ffty = 0, n_fftx = 0;
     int fft_lines = 1, fft_tile = 1, fft_maxK = 0;
+    int fft_plane = 0;              // 1: plane passes (fftz_items = planes, fftx_items = x blocks)
+    size_t fft_plane_lds = 0, fft_x_lds = 0;
     int pme_any = 0;
 
     // energies [nsys][E_NTERMS] in 2^30 fixed point
@@ -102,12 +125,14 @@ struct Batch {
     int timing = 0;                 // NX_TIMING_* mode
 };
 
-template <class P> void launch_nlist(Batch<P>& B);
+template <class P> void launch_nlist(Batch<P>& B, bool clear_flag);
 template <class P> void launch_pack(Batch<P>& B);
 template <class P> void launch_pair(Batch<P>& B, bool energy);
 template <class P> void launch_bonded(Batch<P>& B, int mask, bool energy);
 template <class P> void launch_atom_forces(Batch<P>& B, int mask, bool energy);
 template <class P> void launch_pme_spread(Batch<P>& B);
+bool fft_plane_enabled();
+template <class P> hipError_t prepare_fft(Batch<P>& B);
 template <class P> void launch_fft_forward(Batch<P>& B);
 template <class P> void launch_fft_convolve(Batch<P>& B, bool energy);
 template <class P> void launch_fft_inverse(Batch<P>& B);

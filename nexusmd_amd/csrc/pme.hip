@@ -23,6 +23,8 @@
 // twiddles from a per-axis table exp(-2 pi i t / K) computed on the host in FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "nx_device.h"
@@ -375,12 +377,13 @@ __device__ __forceinline__ void load_twiddles(R2* dst, const R2* __restrict__ sr
     for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
 }
 
-// FFT of `lines` lines of length N stored [line][N] in `a`; `b` is scratch;
-// `tw` holds exp(-2 pi i m / N), m < N, in LDS.  Stockham autosort, radix 4,
-// 2, 3, 5; unnormalised; sign -1 forward, +1 backward.  Returns the buffer
-// holding the result.  All threads of the block call it.
+// FFT of `lines` lines of length N in LDS, point m of line l at a[l * ls + m * es];
+// `b` is scratch of the same layout; `tw` holds exp(-2 pi i m / N), m < N, in
+// LDS.  Stockham autosort, radix 4, 2, 3, 5; unnormalised; sign -1 forward, +1
+// backward.  Returns the buffer holding the result.  All threads of the block
+// call it; every stage ends with a barrier.
 template <typename real, typename R2>
-__device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* tw, int sign) {
+__device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, int ls, int es, const R2* tw, int sign) {
     int Ns = 1;
     int rem = N;
     R2* src = a;
@@ -394,12 +397,12 @@ __device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* tw, int s
         for (int w = threadIdx.x; w < total; w += blockDim.x) {
             const int line = qdiv(w, inv_nb), j = w - line * nb;
             const int jq = qdiv(j, inv_ns), k = j - jq * Ns;
-            const R2* sp = src + line * N;
-            R2* dp = dst + line * N;
+            const R2* sp = src + line * ls;
+            R2* dp = dst + line * ls;
             R2 v[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
-                if (q < r) v[q] = sp[j + q * nb];
+                if (q < r) v[q] = sp[(j + q * nb) * es];
             if (k != 0) {
 #pragma unroll
                 for (int q = 1; q < 5; ++q) {
@@ -414,7 +417,7 @@ __device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, const R2* tw, int s
             const int base = jq * Ns * r + k;
 #pragma unroll
             for (int q = 0; q < 5; ++q)
-                if (q < r) dp[base + q * Ns] = v[q];
+                if (q < r) dp[(base + q * Ns) * es] = v[q];
         }
         __syncthreads();
         R2* t = src;
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, 
         A[i] = v;
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, TW, -1);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, Kz, 1, TW, -1);
     R2* c = cgrid + d.cgrid_off + (long long)line0 * Kzh;
     const float inv_kzh = 1.0f / (float)Kzh;
     for (int i = threadIdx.x; i < nl * Kzh; i += blockDim.x) {
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(256) void fft_z_inv(const SysDev* __restrict__ sd, 
         A[i] = v;
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, TW, +1);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kz, nl, Kz, 1, TW, +1);
     real* o = phi + d.rgrid_off + (long long)line0 * Kz;
     for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) o[i] = res[i].x;
 }
@@ -520,7 +523,7 @@ __global__ __launch_bounds__(256) void fft_y(const SysDev* __restrict__ sd, cons
         A[t * Ky + y] = c[(long long)y * Kzh + t];
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Ky, nt, TW, sign);
+    R2* res = fft_lines_lds<real, R2>(A, B, Ky, nt, Ky, 1, TW, sign);
     for (int i = threadIdx.x; i < nt * Ky; i += blockDim.x) {
         const int y = qdiv(i, inv_nt), t = i - y * nt;
         c[(long long)y * Kzh + t] = res[t * Ky + y];
@@ -556,7 +559,7 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
         A[t * Kx + x] = c[x * sx + t];
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, TW, -1);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, Kx, 1, TW, -1);
     long long e = 0;   // fixed point per element: independent of the tile shape
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
         const int x = qdiv(i, inv_nt), t = i - x * nt;
@@ -573,7 +576,7 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
     }
     __syncthreads();
     R2* other = (res == A) ? B : A;
-    R2* res2 = fft_lines_lds<real, R2>(res, other, Kx, nt, TW, +1);
+    R2* res2 = fft_lines_lds<real, R2>(res, other, Kx, nt, Kx, 1, TW, +1);
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
         const int x = qdiv(i, inv_nt), t = i - x * nt;
         c[x * sx + t] = res2[t * Kx + x];
@@ -607,10 +610,186 @@ __global__ __launch_bounds__(256) void fft_x_plain(const SysDev* __restrict__ sd
         A[t * Kx + x] = c[x * sx + t];
     }
     __syncthreads();
-    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, TW, sign);
+    R2* res = fft_lines_lds<real, R2>(A, B, Kx, nt, Kx, 1, TW, sign);
     for (int i = threadIdx.x; i < nt * Kx; i += blockDim.x) {
         const int x = qdiv(i, inv_nt), t = i - x * nt;
         c[x * sx + t] = res[t * Kx + x];
+    }
+}
+
+// ---------------------------------------------------------- plane FFT passes
+//
+// Whole x-planes in LDS (gfx950 gives a workgroup up to 160 KiB): the forward
+// pass transforms along z and then y in one kernel, the x pass transforms
+// along x, multiplies by the influence function and transforms back, and the
+// inverse pass runs y and then z.  Three launches per step instead of five.
+// The real-to-complex z transform handles two real lines (y = 2p, 2p + 1) as
+// one complex line a + ib and separates them afterwards:
+//   A_k = (Z_k + conj Z_{N-k}) / 2,  B_k = (Z_k - conj Z_{N-k}) / 2i.
+// The complex-to-real inverse packs them the same way: Z_k = A_k + i B_k over
+// the Hermitian extension, then a = Re, b = Im of the inverse transform.
+
+// Forward: plane x of the spread grid -> plane x of the half spectrum (z then y).
+// item = (sys, x)
+template <class P>
+__global__ __launch_bounds__(kPlaneThreads) void fft_plane_fwd(const SysDev* __restrict__ sd,
+                                                              const int4* __restrict__ items,
+                                                              const typename P::facc* __restrict__ rgrid,
+                                                              typename P::real2* cgrid,
+                                                              const typename P::real2* __restrict__ tw) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const SysDev& d = sd[it.x];
+    const int x = it.y;
+    const int Ky = d.grid[1], Kz = d.grid[2], Kzh = Kz / 2 + 1, np = (Ky + 1) >> 1;
+    const int cap = max(np * Kz, Ky * Kzh);
+    R2* TWy = smem;
+    R2* TWz = TWy + Ky;
+    R2* A = TWz + Kz;
+    R2* Bf = A + cap;
+    load_twiddles(TWy, tw + d.tw_off[1], Ky);
+    load_twiddles(TWz, tw + d.tw_off[2], Kz);
+    const typename P::facc* g = rgrid + d.rgrid_off + (long long)x * Ky * Kz;
+    const float inv_kz = 1.0f / (float)Kz;
+    for (int i = threadIdx.x; i < np * Kz; i += blockDim.x) {
+        const int p = qdiv(i, inv_kz), z = i - p * Kz;
+        R2 v;
+        v.x = fixed_to_real(g[(2 * p) * Kz + z], (real)0);
+        v.y = (2 * p + 1 < Ky) ? fixed_to_real(g[(2 * p + 1) * Kz + z], (real)0) : (real)0;
+        A[i] = v;
+    }
+    __syncthreads();
+    R2* Z = fft_lines_lds<real, R2>(A, Bf, Kz, np, Kz, 1, TWz, -1);
+    R2* Q = (Z == A) ? Bf : A;
+    const float inv_kzh = 1.0f / (float)Kzh;
+    for (int i = threadIdx.x; i < np * Kzh; i += blockDim.x) {
+        const int p = qdiv(i, inv_kzh), k = i - p * Kzh;
+        const R2 zk = Z[p * Kz + k], zn = Z[p * Kz + (k == 0 ? 0 : Kz - k)];
+        R2 a, b;
+        a.x = (real)0.5 * (zk.x + zn.x);
+        a.y = (real)0.5 * (zk.y - zn.y);
+        b.x = (real)0.5 * (zk.y + zn.y);
+        b.y = (real)0.5 * (zn.x - zk.x);
+        Q[(2 * p) * Kzh + k] = a;
+        if (2 * p + 1 < Ky) Q[(2 * p + 1) * Kzh + k] = b;
+    }
+    __syncthreads();
+    R2* Y = fft_lines_lds<real, R2>(Q, Z, Ky, Kzh, 1, Kzh, TWy, -1);
+    R2* c = cgrid + d.cgrid_off + (long long)x * Ky * Kzh;       // the plane is contiguous
+    for (int i = threadIdx.x; i < Ky * Kzh; i += blockDim.x) c[i] = Y[i];
+}
+
+// Inverse: plane x of the half spectrum -> plane x of the potential (y then z).
+template <class P>
+__global__ __launch_bounds__(kPlaneThreads) void fft_plane_inv(const SysDev* __restrict__ sd,
+                                                              const int4* __restrict__ items,
+                                                              const typename P::real2* __restrict__ cgrid,
+                                                              typename P::real* phi,
+                                                              const typename P::real2* __restrict__ tw) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const SysDev& d = sd[it.x];
+    const int x = it.y;
+    const int Ky = d.grid[1], Kz = d.grid[2], Kzh = Kz / 2 + 1, np = (Ky + 1) >> 1;
+    const int cap = max(np * Kz, Ky * Kzh);
+    R2* TWy = smem;
+    R2* TWz = TWy + Ky;
+    R2* A = TWz + Kz;
+    R2* Bf = A + cap;
+    load_twiddles(TWy, tw + d.tw_off[1], Ky);
+    load_twiddles(TWz, tw + d.tw_off[2], Kz);
+    const R2* c = cgrid + d.cgrid_off + (long long)x * Ky * Kzh;
+    for (int i = threadIdx.x; i < Ky * Kzh; i += blockDim.x) A[i] = c[i];
+    __syncthreads();
+    R2* Y = fft_lines_lds<real, R2>(A, Bf, Ky, Kzh, 1, Kzh, TWy, +1);
+    R2* W = (Y == A) ? Bf : A;
+    const float inv_kz = 1.0f / (float)Kz;
+    for (int i = threadIdx.x; i < np * Kz; i += blockDim.x) {
+        const int p = qdiv(i, inv_kz), k = i - p * Kz;
+        const bool hi = k >= Kzh;                 // upper half from the Hermitian symmetry
+        const int ks = hi ? Kz - k : k;
+        R2 a = Y[(2 * p) * Kzh + ks];
+        R2 b;
+        if (2 * p + 1 < Ky) b = Y[(2 * p + 1) * Kzh + ks];
+        else { b.x = 0; b.y = 0; }
+        if (ks == 0 || 2 * ks == Kz) { a.y = 0; b.y = 0; }      // DC and Nyquist terms are real
+        if (hi) { a.y = -a.y; b.y = -b.y; }
+        R2 zv;
+        zv.x = a.x - b.y;
+        zv.y = a.y + b.x;
+        W[i] = zv;
+    }
+    __syncthreads();
+    R2* Z = fft_lines_lds<real, R2>(W, Y, Kz, np, Kz, 1, TWz, +1);
+    real* o = phi + d.rgrid_off + (long long)x * Ky * Kz;
+    for (int i = threadIdx.x; i < np * Kz; i += blockDim.x) {
+        const int p = qdiv(i, inv_kz), z = i - p * Kz;
+        o[(2 * p) * Kz + z] = Z[i].x;
+        if (2 * p + 1 < Ky) o[(2 * p + 1) * Kz + z] = Z[i].y;
+    }
+}
+
+// X pass over the lines (y, k0 .. k0 + nk - 1): MODE 0 forward, 1 inverse,
+// 2 forward / influence function / inverse, 3 the same with the energy.
+// item = (sys, y, k0, nk)
+template <class P, int MODE>
+__global__ __launch_bounds__(kPlaneThreads) void fft_x_block(const SysDev* __restrict__ sd,
+                                                            const int4* __restrict__ items,
+                                                            typename P::real2* cgrid,
+                                                            const typename P::real* __restrict__ eta,
+                                                            const typename P::real2* __restrict__ tw,
+                                                            long long* energy) {
+    using real = typename P::real;
+    using R2 = typename P::real2;
+    extern __shared__ unsigned char smem_raw[];
+    R2* smem = reinterpret_cast<R2*>(smem_raw);
+    const int4 it = items[blockIdx.x];
+    const int s = it.x, y = it.y, k0 = it.z, nk = it.w;
+    const SysDev& d = sd[s];
+    const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2], Kzh = Kz / 2 + 1;
+    R2* TW = smem;
+    R2* A = TW + Kx;
+    R2* Bf = A + nk * Kx;
+    load_twiddles(TW, tw + d.tw_off[0], Kx);
+    const long long sx = (long long)Ky * Kzh;
+    R2* c = cgrid + d.cgrid_off + (long long)y * Kzh + k0;
+    const float inv_nk = 1.0f / (float)nk;
+    // A[x * nk + t]: one row of nk wave numbers per x
+    for (int i = threadIdx.x; i < Kx * nk; i += blockDim.x) {
+        const int x = qdiv(i, inv_nk), t = i - x * nk;
+        A[i] = c[x * sx + t];
+    }
+    __syncthreads();
+    R2* R = fft_lines_lds<real, R2>(A, Bf, Kx, nk, 1, nk, TW, MODE == 1 ? +1 : -1);
+    if constexpr (MODE >= 2) {
+        const real* et = eta + d.cgrid_off + (long long)y * Kzh + k0;
+        long long e = 0;   // fixed point per element: independent of the tiling
+        for (int i = threadIdx.x; i < Kx * nk; i += blockDim.x) {
+            const int x = qdiv(i, inv_nk), t = i - x * nk;
+            const real w = et[x * sx + t];
+            R2 v = R[i];
+            if constexpr (MODE == 3) {
+                const int kz = k0 + t;
+                const double wt = (kz == 0 || (2 * kz == Kz)) ? 0.5 : 1.0;
+                e += energy_fixed(wt * (double)w * ((double)v.x * v.x + (double)v.y * v.y));
+            }
+            v.x *= w;
+            v.y *= w;
+            R[i] = v;
+        }
+        __syncthreads();
+        R = fft_lines_lds<real, R2>(R, R == A ? Bf : A, Kx, nk, 1, nk, TW, +1);
+        if constexpr (MODE == 3) block_energy_add(&energy[s * E_NTERMS + E_RECIP], e);
+    }
+    for (int i = threadIdx.x; i < Kx * nk; i += blockDim.x) {
+        const int x = qdiv(i, inv_nk), t = i - x * nk;
+        c[x * sx + t] = R[i];
     }
 }
 
@@ -642,9 +821,33 @@ static size_t fft_smem(const Batch<P>& B, int lines, int N) {
     return (size_t)(2 * lines + 1) * N * sizeof(typename P::real2);   // two line buffers + twiddles
 }
 
+// NEXUSMD_FFT_PLANE=0 selects the five line passes even where planes fit.
+bool fft_plane_enabled() {
+    const char* e = std::getenv("NEXUSMD_FFT_PLANE");
+    return !(e && std::string(e) == "0");
+}
+
+// Raises the dynamic-LDS limit of the plane kernels to what the batch needs.
+template <class P>
+hipError_t prepare_fft(Batch<P>& B) {
+    if (!B.fft_plane) return hipSuccess;
+    const int pl = (int)B.fft_plane_lds, xl = (int)B.fft_x_lds;
+    const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+    hipError_t e = hipFuncSetAttribute((const void*)fft_plane_fwd<P>, a, pl);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)fft_plane_inv<P>, a, pl);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)fft_x_block<P, 2>, a, xl);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)fft_x_block<P, 3>, a, xl);
+    return e;
+}
+
 template <class P>
 void launch_fft_forward(Batch<P>& B) {
     if (!B.pme_any) return;
+    if (B.fft_plane) {
+        fft_plane_fwd<P><<<B.n_fftz, kPlaneThreads, B.fft_plane_lds, B.stream>>>(B.d_sys, B.fftz_items, B.rgrid,
+                                                                               B.cgrid, B.twiddle);
+        return;
+    }
     fft_z_fwd<P><<<B.n_fftz, 256, fft_smem(B, B.fft_lines, B.fft_maxK), B.stream>>>(B.d_sys, B.fftz_items, B.rgrid,
                                                                                    B.cgrid, B.twiddle);
     fft_y<P><<<B.n_ffty, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(B.d_sys, B.ffty_items, B.cgrid,
@@ -654,6 +857,15 @@ void launch_fft_forward(Batch<P>& B) {
 template <class P>
 void launch_fft_convolve(Batch<P>& B, bool energy) {
     if (!B.pme_any) return;
+    if (B.fft_plane) {
+        if (energy)
+            fft_x_block<P, 3><<<B.n_fftx, kXPassThreads, B.fft_x_lds, B.stream>>>(B.d_sys, B.fftx_items, B.cgrid,
+                                                                                B.eta, B.twiddle, B.energy);
+        else
+            fft_x_block<P, 2><<<B.n_fftx, kXPassThreads, B.fft_x_lds, B.stream>>>(B.d_sys, B.fftx_items, B.cgrid,
+                                                                                B.eta, B.twiddle, B.energy);
+        return;
+    }
     if (energy)
         fft_x_conv<P, true><<<B.n_fftx, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(
             B.d_sys, B.fftx_items, B.cgrid, B.eta, B.twiddle, B.fft_tile, B.energy);
@@ -665,6 +877,11 @@ void launch_fft_convolve(Batch<P>& B, bool energy) {
 template <class P>
 void launch_fft_inverse(Batch<P>& B) {
     if (!B.pme_any) return;
+    if (B.fft_plane) {
+        fft_plane_inv<P><<<B.n_fftz, kPlaneThreads, B.fft_plane_lds, B.stream>>>(B.d_sys, B.fftz_items, B.cgrid,
+                                                                               B.phi, B.twiddle);
+        return;
+    }
     fft_y<P><<<B.n_ffty, 256, fft_smem(B, B.fft_tile, B.fft_maxK), B.stream>>>(B.d_sys, B.ffty_items, B.cgrid,
                                                                               B.twiddle, B.fft_tile, +1);
     fft_z_inv<P><<<B.n_fftz, 256, fft_smem(B, B.fft_lines, B.fft_maxK), B.stream>>>(B.d_sys, B.fftz_items, B.cgrid,
@@ -681,6 +898,8 @@ void launch_pme_gather(Batch<P>& B) {
 
 template void launch_pme_spread<PrecFP64>(Batch<PrecFP64>&);
 template void launch_pme_spread<PrecMixed>(Batch<PrecMixed>&);
+template hipError_t prepare_fft<PrecFP64>(Batch<PrecFP64>&);
+template hipError_t prepare_fft<PrecMixed>(Batch<PrecMixed>&);
 template void launch_fft_forward<PrecFP64>(Batch<PrecFP64>&);
 template void launch_fft_forward<PrecMixed>(Batch<PrecMixed>&);
 template void launch_fft_convolve<PrecFP64>(Batch<PrecFP64>&, bool);
@@ -732,6 +951,24 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
     hipMemcpyAsync(dy, yi.data(), yi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dx, xi.data(), xi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
     const size_t sm = (size_t)(2 * T + 1) * maxK * sizeof(R2);
+    // plane passes (as the engine uses them) when the planes fit
+    const bool plane = fft_plane_enabled() && plane_lds_bytes<R2>(ny, nz) <= kMaxBlockLds &&
+                       xpass_lds_bytes<R2>(nx, nzh) <= kMaxBlockLds;
+    std::vector<int4> pi, bi;
+    int4 *dp = nullptr, *db = nullptr;
+    const size_t pl = plane_lds_bytes<R2>(ny, nz), xl = xpass_lds_bytes<R2>(nx, nzh);
+    if (plane) {
+        for (int x = 0; x < nx; ++x) pi.push_back(make_int4(0, x, 0, 0));
+        for (int y = 0; y < ny; ++y) bi.push_back(make_int4(0, y, 0, nzh));
+        hipMalloc(&dp, pi.size() * sizeof(int4));
+        hipMalloc(&db, bi.size() * sizeof(int4));
+        hipMemcpyAsync(dp, pi.data(), pi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(db, bi.data(), bi.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+        hipFuncSetAttribute((const void*)fft_plane_fwd<P>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl);
+        hipFuncSetAttribute((const void*)fft_plane_inv<P>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl);
+        hipFuncSetAttribute((const void*)fft_x_block<P, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)xl);
+        hipFuncSetAttribute((const void*)fft_x_block<P, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)xl);
+    }
     if (forward) {
         std::vector<typename P::facc> h(G);
         for (size_t i = 0; i < G; ++i) {
@@ -739,9 +976,14 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
             else h[i] = in[i];
         }
         hipMemcpyAsync(rg, h.data(), G * sizeof(typename P::facc), hipMemcpyHostToDevice, st);
-        fft_z_fwd<P><<<(unsigned)zi.size(), 256, sm, st>>>(dd, dz, rg, cg, dtw);
-        fft_y<P><<<(unsigned)yi.size(), 256, sm, st>>>(dd, dy, cg, dtw, T, -1);
-        fft_x_plain<P><<<(unsigned)xi.size(), 256, sm, st>>>(dd, dx, cg, dtw, T, -1);
+        if (plane) {
+            fft_plane_fwd<P><<<(unsigned)pi.size(), kPlaneThreads, pl, st>>>(dd, dp, rg, cg, dtw);
+            fft_x_block<P, 0><<<(unsigned)bi.size(), kXPassThreads, xl, st>>>(dd, db, cg, nullptr, dtw, nullptr);
+        } else {
+            fft_z_fwd<P><<<(unsigned)zi.size(), 256, sm, st>>>(dd, dz, rg, cg, dtw);
+            fft_y<P><<<(unsigned)yi.size(), 256, sm, st>>>(dd, dy, cg, dtw, T, -1);
+            fft_x_plain<P><<<(unsigned)xi.size(), 256, sm, st>>>(dd, dx, cg, dtw, T, -1);
+        }
         std::vector<R2> hc(Gh);
         hipMemcpyAsync(hc.data(), cg, Gh * sizeof(R2), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
@@ -750,15 +992,22 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
         std::vector<R2> hc(Gh);
         for (size_t i = 0; i < Gh; ++i) { hc[i].x = (real)in[2 * i]; hc[i].y = (real)in[2 * i + 1]; }
         hipMemcpyAsync(cg, hc.data(), Gh * sizeof(R2), hipMemcpyHostToDevice, st);
-        fft_x_plain<P><<<(unsigned)xi.size(), 256, sm, st>>>(dd, dx, cg, dtw, T, +1);
-        fft_y<P><<<(unsigned)yi.size(), 256, sm, st>>>(dd, dy, cg, dtw, T, +1);
-        fft_z_inv<P><<<(unsigned)zi.size(), 256, sm, st>>>(dd, dz, cg, ph, dtw);
+        if (plane) {
+            fft_x_block<P, 1><<<(unsigned)bi.size(), kXPassThreads, xl, st>>>(dd, db, cg, nullptr, dtw, nullptr);
+            fft_plane_inv<P><<<(unsigned)pi.size(), kPlaneThreads, pl, st>>>(dd, dp, cg, ph, dtw);
+        } else {
+            fft_x_plain<P><<<(unsigned)xi.size(), 256, sm, st>>>(dd, dx, cg, dtw, T, +1);
+            fft_y<P><<<(unsigned)yi.size(), 256, sm, st>>>(dd, dy, cg, dtw, T, +1);
+            fft_z_inv<P><<<(unsigned)zi.size(), 256, sm, st>>>(dd, dz, cg, ph, dtw);
+        }
         std::vector<real> hr(G);
         hipMemcpyAsync(hr.data(), ph, G * sizeof(real), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
         for (size_t i = 0; i < G; ++i) out[i] = hr[i];
     }
     hipFree(dd); hipFree(dtw); hipFree(dz); hipFree(dy); hipFree(dx); hipFree(rg); hipFree(cg); hipFree(ph);
+    if (dp) hipFree(dp);
+    if (db) hipFree(db);
 }
 
 template void fft3d_roundtrip<PrecFP64>(int, int, int, const double*, double*, bool, hipStream_t);
