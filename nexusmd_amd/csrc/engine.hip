@@ -320,6 +320,8 @@ class Engine : public EngineBase {
         B.n_chunk_cap = (int)chunk_off;
         B.n_list_cap = list_off;
         B.n_emask_cap = emask_off;
+        if (emask_off >= (long long)kNoMask)       // mask ids share a list word with the image code
+            throw HipError(NX_ERR_CAPACITY, "exclusion-mask table exceeds 2^27 entries");
         B.rgrid_total = rg_off;
         B.cgrid_total = cg_off;
         B.tw_total = tw_off;
@@ -364,6 +366,10 @@ class Engine : public EngineBase {
         dalloc(&B.slot_wpos, S, o);
         dalloc(&B.chunks, B.n_chunk_cap, o);
         dalloc(&B.chunk_prefix, B.nsys + 1, o);
+        dalloc(&B.chunk_cost, B.n_chunk_cap, o);
+        B.pair_blocks = pair_grid_blocks<P>(B.n_cu, B.n_chunk_cap);
+        B.pair_waves = B.pair_blocks * kPairWaves;
+        dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o);
         dalloc(&B.list, (size_t)B.n_list_cap, o);
         dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o);
         dalloc(&B.rgrid, (size_t)B.rgrid_total, o);
@@ -728,7 +734,7 @@ class Engine : public EngineBase {
     void forces_part(int mask, bool energy, int parts) {
         const bool nb = (mask & NX_FORCE_NONBONDED) != 0;
         if (parts & PART_LIST) {
-            // the prefix kernel of the pair kernel clears the rebuild flags
+            // the pair kernel's first launch clears the rebuild flags
             timed("nlist", [&] { launch_nlist<P>(B, !nb); });
             settle();
         }

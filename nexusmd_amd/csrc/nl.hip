@@ -24,10 +24,11 @@
 // set is the reference's set whatever the buffer (skin) is.
 //
 // Every kernel first checks the system's rebuild flag, which the integrator
-// raises when an atom has moved more than skin/2 since the last build.  The
-// pair kernel's prefix kernel clears it (nl_clear_flag when no pair kernel
-// follows).  Between builds the integrator keeps the slot-order positions
-// current.
+// raises when an atom has moved more than skin/2 since the last build; the
+// last kernel (nl_waves) cuts the chunk ranges of the pair kernel's
+// wavefronts, and the kernel launched before the pair kernel clears the flag
+// (nl_clear_flag when no pair kernel follows).  Between builds the integrator
+// keeps the slot-order positions current.
 #include "nx_device.h"
 #include "nx_engine.h"
 
@@ -165,7 +166,11 @@ __global__ void nl_scatter(const SysDev* sd, SysCount* sc, const int* atom_sys, 
 
 // One block (256 threads) per column: deterministic z order, cluster boxes,
 // super items, and the slot-order data of the force kernels (pos: x, y, z, q;
-// NaN coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).
+// NaN coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).  The
+// mixed path's positions are in the build frame (the wrapped build position
+// plus the displacement since the build), so that the list's per-entry images
+// hold until the next build; the FP64 path keeps the raw coordinates for the
+// reference's per-pair nearest image.
 template <class P>
 __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc, const int* col_sys,
                                                   const int* col_count, int* col_fill, const int* col_slot,
@@ -238,9 +243,15 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
             wrapped(d, x, y, z, w);
             slot_wpos[slot] = make_float4((float)w[0], (float)w[1], (float)w[2], 1.0f);
             typename P::real4 p;
-            p.x = (real)x;
-            p.y = (real)y;
-            p.z = (real)z;
+            if constexpr (P::mixed) {
+                p.x = (real)(float)w[0];
+                p.y = (real)(float)w[1];
+                p.z = (real)(float)w[2];
+            } else {
+                p.x = (real)x;
+                p.y = (real)y;
+                p.z = (real)z;
+            }
             p.w = (real)Q[a];
             pos[slot] = p;
             typename P::real2 l;
@@ -367,7 +378,7 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
                                                           const float4* cl_half, const float4* slot_wpos,
                                                           const int* atom_of_slot, const int* slot_of_atom,
                                                           const int* excl_off, const int* excl_idx, uint2* list,
-                                                          unsigned long long* emask, int4* chunks,
+                                                          unsigned long long* emask, int4* chunks, int* chunk_cost,
                                                           int n_items_total) {
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= n_items_total) return;
@@ -384,6 +395,7 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
 
     __shared__ unsigned int l_ent[kListCap];      // cj | imask << 24
     __shared__ int l_eid[kListCap];
+    __shared__ unsigned char l_img[kListCap];     // j image code
     __shared__ unsigned int x_rec[kExclCap];      // (entry * 8 + i-cluster) << 6 | pair bit
     __shared__ int n_x, id_base, lbase, cbase;
     __shared__ float4 ic[8], ih[8];
@@ -475,7 +487,14 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     if (ne > kListCap) { overflow = true; ne = kListCap; }
 
     // 3: atom pairs, one wavefront per entry, batches of eight entries per
-    // wavefront so that their j loads overlap; the kept mask is written in place
+    // wavefront so that their j loads overlap; the kept mask and the entry's
+    // image code are written in place.  The code's image S comes from the
+    // cluster centres of the first i cluster; it stands for the entry when every
+    // pair of its kept tiles that is within rlist at build has S as its nearest
+    // image (a pair beyond rlist cannot come within the cutoff before the next
+    // build under any image).  In a box shorter than 2 rlist a pair can have two
+    // images within rlist, so every entry there takes images pair by pair.
+    const bool small_box = fminf(Lx, fminf(Ly, Lz)) < 2.0f * sqrtf(rl2) + 1e-3f;
     {
         const int ia = lane >> 3, ja = lane & 7;
         float4 pi[8];
@@ -483,31 +502,43 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         for (int k = 0; k < 8; ++k) pi[k] = ip[k * 8 + ia];
         for (int e0 = wv * 8; e0 < ne; e0 += kPlThreads / 8) {
             unsigned int ent[8];
-            float4 pj[8];
+            float4 pj[8], jcen[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 ent[u] = (e0 + u < ne) ? l_ent[e0 + u] : 0u;
                 pj[u] = slot_wpos[(ent[u] & 0xffffffu) * 8 + ja];
+                jcen[u] = cl_ctr[ent[u] & 0xffffffu];
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int cj = (int)(ent[u] & 0xffffffu);
                 const unsigned int im = ent[u] >> 24;
+                const int k0 = im ? __builtin_ctz(im) : 0;
+                const float Sx = rintf((ic[k0].x - jcen[u].x) * iLx), Sy = rintf((ic[k0].y - jcen[u].y) * iLy),
+                            Sz = rintf((ic[k0].z - jcen[u].z) * iLz);
                 unsigned int keep = 0;
+                bool per_pair = small_box;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     if (im & (1u << k)) {
-                        float dx = pi[k].x - pj[u].x, dy = pi[k].y - pj[u].y, dz = pi[k].z - pj[u].z;
-                        dx -= Lx * rintf(dx * iLx);
-                        dy -= Ly * rintf(dy * iLy);
-                        dz -= Lz * rintf(dz * iLz);
+                        const float ux = pi[k].x - pj[u].x, uy = pi[k].y - pj[u].y, uz = pi[k].z - pj[u].z;
+                        const float nx = rintf(ux * iLx), ny = rintf(uy * iLy), nz = rintf(uz * iLz);
+                        const float dx = ux - Lx * nx, dy = uy - Ly * ny, dz = uz - Lz * nz;
                         // NaN (padding) compares false
                         bool ok = dx * dx + dy * dy + dz * dz < rl2;
                         if (cj == fc + k) ok = ok && ja > ia;
-                        if (__ballot(ok)) keep |= 1u << k;
+                        const bool bad = ok && !(nx == Sx && ny == Sy && nz == Sz);
+                        if (__ballot(ok)) {
+                            keep |= 1u << k;
+                            if (__ballot(bad)) per_pair = true;
+                        }
                     }
                 }
-                if (lane == 0 && e0 + u < ne) l_ent[e0 + u] = (unsigned)cj | (keep << 24);
+                if (lane == 0 && e0 + u < ne) {
+                    l_ent[e0 + u] = (unsigned)cj | (keep << 24);
+                    l_img[e0 + u] = (unsigned char)(per_pair ? kImagePerPair
+                                                            : (unsigned)((Sx + 1.f) * 9.f + (Sy + 1.f) * 3.f + (Sz + 1.f)));
+                }
             }
         }
     }
@@ -517,10 +548,14 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         for (int b = 0; b < ne; b += kPlThreads) {
             const int e = b + tid;
             const unsigned int v = (e < ne) ? l_ent[e] : 0u;
+            const unsigned char img = (e < ne) ? l_img[e] : 0;
             const bool kept = (v >> 24) != 0u;
             int cnt;
             const int pos = nk + block_excl_scan(kept, wsum, cnt);   // every read precedes every write
-            if (kept) l_ent[pos] = v;
+            if (kept) {
+                l_ent[pos] = v;
+                l_img[pos] = img;
+            }
             nk += cnt;
             __syncthreads();
         }
@@ -608,7 +643,8 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     for (int e = tid; e < ne; e += kPlThreads) {
         uint2 v;
         v.x = l_ent[e];
-        v.y = (l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : 0xffffffffu;
+        v.y = ((l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : kNoMask) |
+              ((unsigned)l_img[e] << kEidBits);
         list[d.list_off + lbase + e] = v;
     }
     if (tid == 0) {
@@ -620,11 +656,103 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         const int e0 = d.list_off + lbase + c * kChunkEntries;
         const int e1 = d.list_off + lbase + min(ne, (c + 1) * kChunkEntries);
         chunks[d.chunk_off + cbase + c] = make_int4(fc, ncl, e0, e1);
+        int cost = kCostChunk;
+        for (int e = c * kChunkEntries; e < min(ne, (c + 1) * kChunkEntries); ++e)
+            cost += kCostEntry + kCostTile * __popc(l_ent[e] >> 24);
+        chunk_cost[d.chunk_off + cbase + c] = cost;
     }
 }
 
+// Last kernel of the list work, one block.  After a build in any system of the
+// batch: the batch's chunk order (system by system, prefix[]) and, for the NW
+// wavefronts of the pair kernel, contiguous chunk ranges of near-equal cost:
+// chunk g goes to wavefront floor(NW * (midpoint of g's cost interval) / total
+// cost), so wave_start[w] is the first chunk whose midpoint reaches w / NW of
+// the total.
+__global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost,
+                                                 int* prefix, int* wave_start, int NW) {
+    __shared__ int any, total;
+    __shared__ long long wsum[16];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        int a = 0;
+        for (int s = 0; s < nsys; ++s) a |= sc[s].rebuild;
+        if (a) {
+            int t = 0;
+            for (int s = 0; s < nsys; ++s) {
+                prefix[s] = t;
+                t += min(sc[s].n_chunks, sd[s].chunk_cap);     // the counter runs past the capacity on overflow
+            }
+            prefix[nsys] = t;
+            total = t;
+        }
+        any = a;
+    }
+    __syncthreads();
+    if (!any) return;
+    const int T = total;
+    auto sys_of = [&](int g) {          // largest s with prefix[s] <= g
+        int lo = 0, hi = nsys - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    };
+    auto cost_of = [&](int g, int& s) {
+        while (g >= prefix[s + 1]) ++s;
+        return (long long)chunk_cost[sd[s].chunk_off + g - prefix[s]];
+    };
+    const int per = (T + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int g_begin = min(T, tid * per), g_end = min(T, g_begin + per);
+    long long mine = 0;
+    int s = (g_begin < g_end) ? sys_of(g_begin) : 0;
+    for (int g = g_begin; g < g_end; ++g) mine += cost_of(g, s);
+    // block exclusive scan of the per-thread sums
+    long long incl = mine;
+    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    long long base = incl - mine, ctot = 0;
+    for (int w2 = 0; w2 < (int)(blockDim.x >> 6); ++w2) {
+        if (w2 < wv) base += wsum[w2];
+        ctot += wsum[w2];
+    }
+    if (T == 0) {
+        for (int w = tid; w <= NW; w += blockDim.x) wave_start[w] = 0;
+    } else if (g_begin < g_end) {
+        // doubled units: M_g = 2 * (cost before g) + cost of g; wave of g = M_g * NW / (2 * ctot)
+        auto wave_of = [&](long long M) { return (int)min((long long)NW - 1, M * NW / (2 * ctot)); };
+        int a_prev = -1;
+        if (g_begin > 0) {
+            int sp = sys_of(g_begin - 1);
+            const long long cp = cost_of(g_begin - 1, sp);
+            a_prev = wave_of(2 * (base - cp) + cp);
+        }
+        long long cum = base;
+        s = sys_of(g_begin);
+        int a = a_prev;
+        for (int g = g_begin; g < g_end; ++g) {
+            const long long c = cost_of(g, s);
+            a = wave_of(2 * cum + c);
+            for (int w = a_prev + 1; w <= a; ++w) wave_start[w] = g;
+            a_prev = a;
+            cum += c;
+        }
+        if (g_end == T)
+            for (int w = a + 1; w <= NW; ++w) wave_start[w] = T;
+    }
+}
+
+// Clears the rebuild flags when no pair kernel follows (its first launch does
+// it otherwise) and counts the builds.
 __global__ void nl_clear_flag(SysCount* sc, int nsys) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     if (sc[s].rebuild) {
         sc[s].rebuild = 0;
@@ -633,7 +761,7 @@ __global__ void nl_clear_flag(SysCount* sc, int nsys) {
 }
 
 template <class P>
-void launch_nlist(Batch<P>& B, bool clear_flag) {
+void launch_nlist(Batch<P>& B, bool clear_flags) {
     const int T = 256;
     hipStream_t st = B.stream;
     auto nb = [](long n, int t) { return (unsigned)((n + t - 1) / t); };
@@ -648,8 +776,9 @@ void launch_nlist(Batch<P>& B, bool clear_flag) {
                                             B.slot_zkey, B.slot_wpos, B.pos, B.lj, B.XB, B.YB, B.ZB);
     nl_pairlist<<<B.n_super_cap, kPlThreads, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
                                               B.cl_ctr, B.cl_half, B.slot_wpos, B.atom_of_slot, B.slot_of_atom, B.excl_off,
-                                              B.excl_idx, B.list, B.emask, B.chunks, B.n_super_cap);
-    if (clear_flag) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
+                                              B.excl_idx, B.list, B.emask, B.chunks, B.chunk_cost, B.n_super_cap);
+    nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves);
+    if (clear_flags) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
 }
 
 template void launch_nlist<PrecFP64>(Batch<PrecFP64>&, bool);

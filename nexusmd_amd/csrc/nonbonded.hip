@@ -100,26 +100,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Chunk prefix over systems: work item g of the whole batch maps to system s
-// with prefix[s] <= g < prefix[s+1] and chunk sd[s].chunk_off + g - prefix[s].
-// Also ends the step's list work: clears the rebuild flags (the list kernels
-// that read them ran before) and counts the builds.
-__global__ void pair_prefix_kernel(const SysDev* __restrict__ sd, SysCount* sc, int nsys, int* prefix) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int t = 0;
-    for (int s = 0; s < nsys; ++s) {
-        prefix[s] = t;
-        // the counter runs past the capacity when a build overflows
-        t += min(sc[s].n_chunks, sd[s].chunk_cap);
-        if (sc[s].rebuild) {
-            sc[s].rebuild = 0;
-            sc[s].rebuilds += 1;
-        }
-    }
-    prefix[nsys] = t;
-}
-
-constexpr int kPairWaves = 4;   // independent wavefronts per 256-thread block
 
 // Lanes l = 8 * ia + ja with ja <= ia: on a diagonal tile (i and j the same
 // cluster) these are the self pairs and the half counted from the other side.
@@ -132,8 +112,9 @@ constexpr unsigned long long lower_triangle_mask() {
 constexpr unsigned long long kLowerTriangle = lower_triangle_mask();
 
 template <class P, bool ENERGY>
-__global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd, const int* __restrict__ prefix,
-                                                   int nsys, const int4* __restrict__ chunks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_waves_per_simd))) void pair_kernel(const SysDev* __restrict__ sd, const int* __restrict__ prefix,
+                                                   int nsys, const int* __restrict__ wave_start,
+                                                   const int4* __restrict__ chunks,
                                                    const uint2* __restrict__ list,
                                                    const unsigned long long* __restrict__ emask,
                                                    const typename P::real4* __restrict__ pos,
@@ -152,12 +133,10 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
     real* jf = jf_s[w];
     int* jc = jc_s[w];
 
-    // contiguous range of work items per wavefront; blocks of one XCD take adjacent ranges
-    const int total = prefix[nsys];
-    const int NW = gridDim.x * kPairWaves;
+    // contiguous range of chunks per wavefront, cut to equal estimated work by
+    // the list build; blocks of one XCD take adjacent ranges
     const int vw = xcd_remap(blockIdx.x, gridDim.x) * kPairWaves + w;
-    const int per = (total + NW - 1) / NW;
-    const int g0 = vw * per, g1 = min(total, g0 + per);
+    const int g0 = wave_start[vw], g1 = wave_start[vw + 1];
     const real ke = (real)kOne4PiEps0;
 
     // Everything a chunk needs from memory before its first pair: the chunk
@@ -231,14 +210,15 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
             // every lane loaded the same list word: move it to scalar registers so the
             // per-cluster tests below are scalar branches
             const unsigned ex = __builtin_amdgcn_readfirstlane(ent.x);
-            const unsigned eid = __builtin_amdgcn_readfirstlane(ent.y);
+            const unsigned ey = __builtin_amdgcn_readfirstlane(ent.y);
+            const unsigned eid = ey & kNoMask, img = ey >> kEidBits;
             const int cj = (int)(ex & 0xffffffu);
             const unsigned im = ex >> 24;
             // exclusion bits of the 8 cluster pairs, one load before the cluster loop
             unsigned long long em[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) em[k] = 0ull;
-            if (eid != 0xffffffffu) {
+            if (eid != kNoMask) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) em[k] = emask[(size_t)eid * 8 + k];
             }
@@ -252,6 +232,20 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
                 lj_n = lj[sjn];
             }
             real fxj = 0, fyj = 0, fzj = 0;
+            // mixed path: j moved to the entry's image once (build frame), so the
+            // tiles need no nearest-image step unless the entry takes images pair
+            // by pair; the FP64 path keeps the reference's per-pair correction
+            real4 pjs = pj;
+            bool per_pair = true;
+            if constexpr (P::mixed) {
+                if (img != kImagePerPair) {
+                    per_pair = false;
+                    const int c = (int)img;
+                    pjs.x += Lx * (real)(c / 9 - 1);
+                    pjs.y += Ly * (real)((c / 3) % 3 - 1);
+                    pjs.z += Lz * (real)(c % 3 - 1);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if (im & (1u << k)) {
@@ -262,15 +256,19 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
                     const real2 li = ilj[k * 8 + ia];
                     real dx, dy, dz;
                     if constexpr (P::mixed) {
-                        // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
-                        const real ux = pi.x - pj.x, uy = pi.y - pj.y, uz = pi.z - pj.z;
-                        dx = ux - Lx * rintf(ux * iLx);
-                        dy = uy - Ly * rintf(uy * iLy);
-                        dz = uz - Lz * rintf(uz * iLz);
+                        dx = pi.x - pjs.x;
+                        dy = pi.y - pjs.y;
+                        dz = pi.z - pjs.z;
+                        if (per_pair) {
+                            // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
+                            dx -= Lx * rintf(dx * iLx);
+                            dy -= Ly * rintf(dy * iLy);
+                            dz -= Lz * rintf(dz * iLz);
+                        }
                     } else {
-                        dx = min_image1(pi.x - pj.x, Lx);
-                        dy = min_image1(pi.y - pj.y, Ly);
-                        dz = min_image1(pi.z - pj.z, Lz);
+                        dx = min_image1(pi.x - pjs.x, Lx);
+                        dy = min_image1(pi.y - pjs.y, Ly);
+                        dz = min_image1(pi.z - pjs.z, Lz);
                     }
                     real r2;
                     if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
@@ -279,14 +277,14 @@ __global__ __launch_bounds__(256) void pair_kernel(const SysDev* __restrict__ sd
                     real ecv, elv, dEdR;
                     if constexpr (P::mixed) {
                         // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
-                        dEdR = pair_mixed(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on, sw_inv,
+                        dEdR = pair_mixed(r2, pi.w * pjs.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on, sw_inv,
                                           ecv, elv);
                         dEdR = on ? dEdR : 0.0f;
                     } else {
                         dEdR = 0;
                         ecv = elv = 0;
                         if (on)
-                            dEdR = pair_fp64(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
+                            dEdR = pair_fp64(r2, pi.w * pjs.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
                                              sw_inv, ecv, elv);
                     }
                     // off lanes may hold NaN displacements (padding slots): select both factors
@@ -388,7 +386,8 @@ __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, cons
             if (!(im & (1u << k))) continue;
             const int ai = atom_of_slot[(fc + k) * 8 + ia];
             bool excl = (cj == fc + k) && (ja <= ia);
-            if (ent.y != 0xffffffffu) excl |= ((emask[(size_t)ent.y * 8 + k] >> lane) & 1ull) != 0ull;
+            if ((ent.y & kNoMask) != kNoMask)
+                excl |= ((emask[(size_t)(ent.y & kNoMask) * 8 + k] >> lane) & 1ull) != 0ull;
             if (excl || ai < 0 || aj < 0) continue;
             const double dx = min_image1(X[ai] - X[aj], Lx);
             const double dy = min_image1(Y[ai] - Y[aj], Ly);
@@ -402,32 +401,51 @@ __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, cons
 }
 
 // Blocks of the pair kernel one CU holds at once (its occupancy limit).
-template <class P, bool ENERGY>
+template <class P>
 static int pair_blocks_per_cu() {
     static const int v = [] {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_kernel<P, ENERGY>, 256, 0) != hipSuccess || n <= 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_kernel<P, false>, 256, 0) != hipSuccess || n <= 0)
             n = 2;
         return n;
     }();
     return v;
 }
 
+// Persistent grid of the pair kernel, all blocks resident at once; where a CU
+// holds four or more, one block's worth is left to the PME kernels beside it.
+// Fixed per batch layout: the list build cuts the chunk ranges for it.
+template <class P>
+int pair_grid_blocks(int n_cu, int n_chunk_cap) {
+    const int occ = pair_blocks_per_cu<P>();
+    const int per_cu = occ >= 4 ? occ - 1 : occ;
+    return std::max(1, std::min((n_chunk_cap + kPairWaves - 1) / kPairWaves, per_cu * std::max(1, n_cu)));
+}
+
+// Ends the step's list work before the pair kernel: clears the rebuild flags
+// (every list kernel that reads them ran before) and counts the builds.  As a
+// separate launch it also gives the kernels forked to the side stream (bonded
+// terms, PME) a head start on the CUs before the pair kernel's persistent grid
+// takes its slots; without it the step measured about 10% slower.
+__global__ void pair_begin_kernel(SysCount* sc, int nsys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    if (sc[s].rebuild) {
+        sc[s].rebuild = 0;
+        sc[s].rebuilds += 1;
+    }
+}
+
 template <class P>
 void launch_pair(Batch<P>& B, bool energy) {
+    pair_begin_kernel<<<(B.nsys + 63) / 64, 64, 0, B.stream>>>(B.d_cnt, B.nsys);
     if (B.n_chunk_cap == 0) return;
-    pair_prefix_kernel<<<1, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_prefix);
-    // persistent grid, all blocks resident at once; where a CU holds four or
-    // more, one block's worth is left to the PME kernels running beside
-    const int occ = energy ? pair_blocks_per_cu<P, true>() : pair_blocks_per_cu<P, false>();
-    const int per_cu = occ >= 4 ? occ - 1 : occ;
-    const unsigned blocks = (unsigned)std::max(
-        1, std::min((B.n_chunk_cap + kPairWaves - 1) / kPairWaves, per_cu * std::max(1, B.n_cu)));
+    const unsigned blocks = (unsigned)B.pair_blocks;
     if (energy)
-        pair_kernel<P, true><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.chunks, B.list,
+        pair_kernel<P, true><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
                                                            B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);
     else
-        pair_kernel<P, false><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.chunks, B.list,
+        pair_kernel<P, false><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
                                                             B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);
 }
 
@@ -444,6 +462,8 @@ long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* 
     return n;
 }
 
+template int pair_grid_blocks<PrecFP64>(int, int);
+template int pair_grid_blocks<PrecMixed>(int, int);
 template void launch_pair<PrecFP64>(Batch<PrecFP64>&, bool);
 template void launch_pair<PrecMixed>(Batch<PrecMixed>&, bool);
 template long long launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);

@@ -67,6 +67,21 @@ struct SysCount {
 };
 
 constexpr int kChunkEntries = 32;       // pair-list entries per pair-kernel wavefront
+constexpr int kPairWaves = 4;           // independent wavefronts per 256-thread pair-kernel block
+// Estimated pair-kernel work of a chunk in quarter tiles: per tile, per list
+// entry and per chunk (i staging and the force flush).  The list build writes
+// it per chunk; the wavefronts' contiguous chunk ranges are cut to equal sums.
+constexpr int kCostTile = 4, kCostEntry = 2, kCostChunk = 16;
+
+// Pair-list entry: x = j cluster | i-cluster mask << 24; y = exclusion-mask id
+// in the low kEidBits (kNoMask: no excluded pair in the entry's tiles) | the
+// j image code << kEidBits.  Image code (sx+1)*9 + (sy+1)*3 + (sz+1): every
+// pair of the entry's tiles that can come within the cutoff before the next
+// build has its nearest image at j + (sx Lx, sy Ly, sz Lz) in the build frame;
+// kImagePerPair: take the nearest image pair by pair.
+constexpr unsigned kEidBits = 27;
+constexpr unsigned kNoMask = (1u << kEidBits) - 1u;
+constexpr unsigned kImagePerPair = 31u;
 
 // ------------------------------------------------------------ precision
 
@@ -76,6 +91,7 @@ struct PrecFP64 {
     using real4 = double4;
     using facc = double;          // force accumulator element
     static constexpr bool mixed = false;
+    static constexpr int pair_waves_per_simd = 1;   // minimum pair-kernel waves per SIMD the registers must allow
 };
 
 struct PrecMixed {
@@ -84,6 +100,7 @@ struct PrecMixed {
     using real4 = float4;
     using facc = long long;       // 2^32 fixed point
     static constexpr bool mixed = true;
+    static constexpr int pair_waves_per_simd = 4;
 };
 
 __device__ __forceinline__ long long to_fixed(float v) {

@@ -76,6 +76,9 @@ struct Batch {
     int* chunk_sys = nullptr;
     int4* chunks = nullptr;         // (first cluster, n clusters, first entry, end entry)
     int* chunk_prefix = nullptr;    // [nsys + 1] running total of chunks over systems
+    int* chunk_cost = nullptr;      // per chunk: estimated pair-kernel work (list build)
+    int* wave_start = nullptr;      // [pair_waves + 1] first chunk of each pair-kernel wavefront
+    int pair_blocks = 0, pair_waves = 0;
     int n_cu = 0;                   // compute units of the device
     int n_chunk_cap = 0;
     uint2* list = nullptr;
@@ -125,7 +128,8 @@ struct Batch {
     int timing = 0;                 // NX_TIMING_* mode
 };
 
-template <class P> void launch_nlist(Batch<P>& B, bool clear_flag);
+template <class P> void launch_nlist(Batch<P>& B, bool clear_flags);
+template <class P> int pair_grid_blocks(int n_cu, int n_chunk_cap);
 template <class P> void launch_pack(Batch<P>& B);
 template <class P> void launch_pair(Batch<P>& B, bool energy);
 template <class P> void launch_bonded(Batch<P>& B, int mask, bool energy);
