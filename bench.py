@@ -8,9 +8,10 @@ plus the leapfrog update, with the neighbour list rebuilt on the device
 whenever an atom has moved more than half the buffer.
 
 Prints ONE JSON line.  `value` is the mixed-precision ns/day over the timed
-region (step graphs replayed, HIP events bracketing the pair kernel of every
-step); the FP64 path, the force errors of both paths against the FP64 CPU
-oracle on identical positions, a per-kernel breakdown, the pair-kernel
+region (step graphs replayed; every 8th step replays as graphs around the pair
+kernel with HIP events bracketing it); the FP64 path, the force errors of both
+paths against the FP64 CPU oracle on identical positions, a per-kernel
+breakdown, the pair-kernel
 roofline, a CPU baseline (the oracle, one core, a bounded sample of the same
 workload) and the batched-serving measurement (256 ligand-in-water requests,
 continuous batching vs one at a time, p50/p99 step latency) ride along.
@@ -86,10 +87,11 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
 
 def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
     """Warm-up, then the timed region: exactly `steps` steps replayed as step
-    graphs, with HIP events on the context's stream bracketing the pair kernel
-    of every step (nx_step returns after the stream has drained, so the wall
-    clock around it covers exactly the timed steps).  Before it, the same count
-    as single-graph replays without events; after it, a shorter run with events
+    graphs, every 8th of them as graphs around the pair kernel with HIP events
+    on the context's stream bracketing it, and an event closing every step
+    (nx_step returns after the stream has drained, so the wall clock around it
+    covers exactly the timed steps).  Before it, the same count as
+    single-graph replays without events; after it, a shorter run with events
     around every kernel family for the per-kernel breakdown."""
     from nexusmd_amd import _lib as L
     from nexusmd_amd import engine as E
@@ -120,7 +122,9 @@ def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
         kern[name] = {"ms_per_launch": ms, "launches": n}
     ctx.enable_timing(L.TIMING_OFF)
     timed = {"pair_ms_per_launch": pair_ms, "pair_launches": pair_n, "step_ms_events": step_ms,
-             "breakdown_steps": nb, "launch": "step graphs" if graphs else "direct launches"}
+             "breakdown_steps": nb,
+             "launch": ("step graphs; every 8th step as graphs around the pair kernel with events on it"
+                        if graphs else "direct launches, events around every kernel family")}
     return ctx, (t1 - t0), kern, stats, 0.5 * (pairs0 + pairs1), graph_s, timed
 
 

@@ -139,10 +139,12 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
  *                         the other on one stream, with events around it:
  *                         "nlist", "pack", "bonded", "atom_forces", "pair",
  *                         "pme_spread", "fft", "pme_gather", "integrate"
- *   NX_TIMING_GRAPH_PAIR  nx_step replays each step as graphs around the pair
- *                         kernel (the terms and PME beside it, the update and
- *                         the next list after it) with events on the main
- *                         stream around the pair kernel: "pair" and "step"
+ *   NX_TIMING_GRAPH_PAIR  nx_step replays every step as one graph, except
+ *                         every 8th (the first included), which replays as
+ *                         graphs around the pair kernel (list before it, terms
+ *                         and PME beside it, update after it) with events on
+ *                         the main stream around the pair kernel; an event
+ *                         closes every step: "pair" (sampled steps) and "step"
  * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
  * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);
  * NEXUSMD_OVERLAP=0 keeps every kernel on one stream.

@@ -813,30 +813,38 @@ class Engine : public EngineBase {
         begin_call();
         settle();
         if (B.timing == NX_TIMING_GRAPH_PAIR && nsteps > 0 && graphs_enabled_) {
-            // graphs around the pair kernel, with events on the main stream around
-            // it: [list, positions] once, then per step the side graph (terms,
-            // PME; on the side stream beside the pair kernel), the pair kernel and
-            // [update, list, positions] ([update] after the last step)
+            // Every kPairSampleEvery-th step (the first included) replays as graphs
+            // around the pair kernel -- [list, positions], the side graph (terms,
+            // PME; on the side stream beside the pair kernel), the pair kernel,
+            // [update] -- with events on the main stream around the pair kernel;
+            // the other steps replay the whole step as one graph.  An event on the
+            // main stream closes every step.
             if (!seg_exec_[0] || graph_dt_seg_ != dt || graph_mask_seg_ != mask) build_segments(dt, mask);
+            if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
             const bool side = use_side(mask);
             hipEvent_t e0 = ev();
             HIPCHK(hipEventRecord(e0, B.stream));
-            HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
             for (int i = 0; i < nsteps; ++i) {
-                hipEvent_t a = ev(), b = ev(), c = ev();
-                if (side) {
-                    fork_side();
-                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], side_));
+                hipEvent_t c = ev();
+                if (i % kPairSampleEvery == 0) {
+                    hipEvent_t a = ev(), b = ev();
+                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
+                    if (side) {
+                        fork_side();
+                        HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], side_));
+                    } else {
+                        HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
+                    }
+                    HIPCHK(hipEventRecord(a, B.stream));
+                    if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
+                    HIPCHK(hipEventRecord(b, B.stream));
+                    if (side) join_side();
+                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_LAST], B.stream));
+                    evs_.push_back({"pair", a, b});
                 } else {
-                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
+                    HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
                 }
-                HIPCHK(hipEventRecord(a, B.stream));
-                if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
-                HIPCHK(hipEventRecord(b, B.stream));
-                if (side) join_side();
-                HIPCHK(hipGraphLaunch(seg_exec_[i + 1 < nsteps ? SEG_MID : SEG_LAST], B.stream));
                 HIPCHK(hipEventRecord(c, B.stream));
-                evs_.push_back({"pair", a, b});
                 evs_.push_back({"step", e0, c});
                 e0 = c;
             }
@@ -878,8 +886,8 @@ class Engine : public EngineBase {
         for (int g = 0; g < SEG_COUNT; ++g) {
             if (g == SEG_PAIR && !(mask & NX_FORCE_NONBONDED)) continue;     // no pair graph
             HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-            if (g == SEG_MID || g == SEG_LAST) launch_integrate<P>(B, dt);
-            if (g == SEG_FIRST || g == SEG_MID) forces_part(mask, false, PART_LIST);
+            if (g == SEG_LAST) launch_integrate<P>(B, dt);
+            if (g == SEG_FIRST) forces_part(mask, false, PART_LIST);
             if (g == SEG_SIDE) side_work(mask, false);
             if (g == SEG_PAIR) launch_pair<P>(B, false);
             hipError_t e = hipStreamEndCapture(B.stream, &seg_[g]);
@@ -1120,9 +1128,9 @@ class Engine : public EngineBase {
     int graph_mask_ = -1;
     int failed_sys_ = -1;
     int grow_depth_ = 0;
-    // timing segments: [list, positions], [terms, PME], [pair], [update, list,
-    // positions], [update]
-    enum { SEG_FIRST, SEG_SIDE, SEG_PAIR, SEG_MID, SEG_LAST, SEG_COUNT };
+    // timing segments: [list, positions], [terms, PME], [pair], [update]
+    enum { SEG_FIRST, SEG_SIDE, SEG_PAIR, SEG_LAST, SEG_COUNT };
+    static constexpr int kPairSampleEvery = 8;
     hipGraph_t seg_[SEG_COUNT] = {};
     hipGraphExec_t seg_exec_[SEG_COUNT] = {};
     hipStream_t side_ = nullptr;            // PME beside the pair kernel
