@@ -33,12 +33,14 @@ namespace nx {
     } while (0)
 
 template <class T>
-static void dalloc(T** p, size_t n, std::vector<void*>& owned) {
+static void dalloc(T** p, size_t n, std::vector<void*>& owned, hipStream_t st) {
     if (n == 0) n = 1;
     HIPCHK(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
     owned.push_back(*p);
-    // zeroed, so capacity-overflow gaps in work arrays read as empty items
-    HIPCHK(hipMemset(*p, 0, n * sizeof(T)));
+    // zeroed, so capacity-overflow gaps in work arrays read as empty items; on the
+    // engine's stream, which does not order with the null stream, so that the
+    // clear lands before the uploads and kernels queued after it
+    HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), st));
 }
 
 template <class T>
@@ -320,8 +322,6 @@ class Engine : public EngineBase {
         B.n_chunk_cap = (int)chunk_off;
         B.n_list_cap = list_off;
         B.n_emask_cap = emask_off;
-        if (emask_off >= (long long)kNoMask)       // mask ids share a list word with the image code
-            throw HipError(NX_ERR_CAPACITY, "exclusion-mask table exceeds 2^27 entries");
         B.rgrid_total = rg_off;
         B.cgrid_total = cg_off;
         B.tw_total = tw_off;
@@ -335,53 +335,53 @@ class Engine : public EngineBase {
 
     void allocate() {
         auto& o = owned_;
-        dalloc(&B.d_sys, B.nsys, o);
-        dalloc(&B.d_cnt, B.nsys, o);
+        dalloc(&B.d_sys, B.nsys, o, B.stream);
+        dalloc(&B.d_cnt, B.nsys, o, B.stream);
         const size_t A = B.n_atoms, S = B.n_slots_cap;
         for (double** p : {&B.X, &B.Y, &B.Z, &B.VX, &B.VY, &B.VZ, &B.invm, &B.mass, &B.q, &B.sig, &B.eps, &B.XB, &B.YB,
                            &B.ZB})
-            dalloc(p, A, o);
-        dalloc(&B.atom_sys, A, o);
-        dalloc(&B.slot_of_atom, A, o);
-        dalloc(&B.atom_col, A, o);
-        dalloc(&B.FAx, A, o);
-        dalloc(&B.FAy, A, o);
-        dalloc(&B.FAz, A, o);
-        dalloc(&B.slot_sys, S, o);
-        dalloc(&B.atom_of_slot, S, o);
-        dalloc(&B.slot_tmp, S, o);
-        dalloc(&B.pos, S, o);
-        dalloc(&B.lj, S, o);
-        dalloc(&B.FSx, S, o);
-        dalloc(&B.FSy, S, o);
-        dalloc(&B.FSz, S, o);
+            dalloc(p, A, o, B.stream);
+        dalloc(&B.atom_sys, A, o, B.stream);
+        dalloc(&B.slot_of_atom, A, o, B.stream);
+        dalloc(&B.atom_col, A, o, B.stream);
+        dalloc(&B.FAx, A, o, B.stream);
+        dalloc(&B.FAy, A, o, B.stream);
+        dalloc(&B.FAz, A, o, B.stream);
+        dalloc(&B.slot_sys, S, o, B.stream);
+        dalloc(&B.atom_of_slot, S, o, B.stream);
+        dalloc(&B.slot_tmp, S, o, B.stream);
+        dalloc(&B.pos, S, o, B.stream);
+        dalloc(&B.lj, S, o, B.stream);
+        dalloc(&B.FSx, S, o, B.stream);
+        dalloc(&B.FSy, S, o, B.stream);
+        dalloc(&B.FSz, S, o, B.stream);
         for (int** p : {&B.col_sys, &B.col_count, &B.col_fill, &B.col_slot, &B.col_super, &B.col_tmp})
-            dalloc(p, B.n_cols, o);
-        dalloc(&B.cl_ctr, S / 8 + 1, o);
-        dalloc(&B.cl_half, S / 8 + 1, o);
-        dalloc(&B.super_sys, B.n_super_cap, o);
-        dalloc(&B.super_item, B.n_super_cap, o);
-        dalloc(&B.chunk_sys, B.n_chunk_cap, o);
-        dalloc(&B.slot_zkey, S, o);
-        dalloc(&B.slot_wpos, S, o);
-        dalloc(&B.chunks, B.n_chunk_cap, o);
-        dalloc(&B.chunk_prefix, B.nsys + 1, o);
-        dalloc(&B.chunk_cost, B.n_chunk_cap, o);
+            dalloc(p, B.n_cols, o, B.stream);
+        dalloc(&B.cl_ctr, S / 8 + 1, o, B.stream);
+        dalloc(&B.cl_half, S / 8 + 1, o, B.stream);
+        dalloc(&B.super_sys, B.n_super_cap, o, B.stream);
+        dalloc(&B.super_item, B.n_super_cap, o, B.stream);
+        dalloc(&B.chunk_sys, B.n_chunk_cap, o, B.stream);
+        dalloc(&B.slot_zkey, S, o, B.stream);
+        dalloc(&B.slot_wpos, S, o, B.stream);
+        dalloc(&B.chunks, B.n_chunk_cap, o, B.stream);
+        dalloc(&B.chunk_prefix, B.nsys + 1, o, B.stream);
+        dalloc(&B.chunk_cost, B.n_chunk_cap, o, B.stream);
         B.pair_blocks = pair_grid_blocks<P>(B.n_cu, B.n_chunk_cap);
         B.pair_waves = B.pair_blocks * kPairWaves;
-        dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o);
-        dalloc(&B.list, (size_t)B.n_list_cap, o);
-        dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o);
-        dalloc(&B.rgrid, (size_t)B.rgrid_total, o);
-        dalloc(&B.cgrid, (size_t)B.cgrid_total, o);
-        dalloc(&B.phi, (size_t)B.rgrid_total, o);
-        dalloc(&B.eta, (size_t)B.cgrid_total, o);
-        dalloc(&B.twiddle, (size_t)B.tw_total, o);
-        dalloc(&B.energy, (size_t)B.nsys * E_NTERMS, o);
-        dalloc(&d_count_, 1, o);
-        if (B.n_bonds) { dalloc(&B.bond_idx, B.n_bonds, o); dalloc(&B.bond_par, B.n_bonds, o); }
-        if (B.n_angles) { dalloc(&B.angle_idx, B.n_angles, o); dalloc(&B.angle_par, B.n_angles, o); }
-        if (B.n_tors) { dalloc(&B.tors_idx, B.n_tors, o); dalloc(&B.tors_par, B.n_tors, o); }
+        dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o, B.stream);
+        dalloc(&B.list, (size_t)B.n_list_cap, o, B.stream);
+        dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o, B.stream);
+        dalloc(&B.rgrid, (size_t)B.rgrid_total, o, B.stream);
+        dalloc(&B.cgrid, (size_t)B.cgrid_total, o, B.stream);
+        dalloc(&B.phi, (size_t)B.rgrid_total, o, B.stream);
+        dalloc(&B.eta, (size_t)B.cgrid_total, o, B.stream);
+        dalloc(&B.twiddle, (size_t)B.tw_total, o, B.stream);
+        dalloc(&B.energy, (size_t)B.nsys * E_NTERMS, o, B.stream);
+        dalloc(&d_count_, 1, o, B.stream);
+        if (B.n_bonds) { dalloc(&B.bond_idx, B.n_bonds, o, B.stream); dalloc(&B.bond_par, B.n_bonds, o, B.stream); }
+        if (B.n_angles) { dalloc(&B.angle_idx, B.n_angles, o, B.stream); dalloc(&B.angle_par, B.n_angles, o, B.stream); }
+        if (B.n_tors) { dalloc(&B.tors_idx, B.n_tors, o, B.stream); dalloc(&B.tors_par, B.n_tors, o, B.stream); }
         HIPCHK(hipDeviceSynchronize());
     }
 
@@ -539,23 +539,23 @@ class Engine : public EngineBase {
             for (int a = 0; a < B.n_atoms; ++a) cnt[a + 1] += cnt[a];
             std::vector<int> at_idx(std::max(1, B.n_tslots)), fill(cnt.begin(), cnt.end() - 1);
             each([&](int a, int slot) { at_idx[fill[a]++] = slot; });
-            dalloc(&B.at_off, cnt.size(), owned_);
-            dalloc(&B.at_idx, at_idx.size(), owned_);
-            dalloc(&B.tF, (size_t)3 * std::max(1, B.n_tslots), owned_);
+            dalloc(&B.at_off, cnt.size(), owned_, B.stream);
+            dalloc(&B.at_idx, at_idx.size(), owned_, B.stream);
+            dalloc(&B.tF, (size_t)3 * std::max(1, B.n_tslots), owned_, B.stream);
             upload(B.at_off, cnt, st);
             upload(B.at_idx, at_idx, st);
         }
         B.n_excl_pairs = (int)epairs.size();
-        dalloc(&B.excl_off, eoff.size(), owned_);
-        dalloc(&B.excl_idx, std::max<size_t>(1, eidx.size()), owned_);
-        if (B.n_excl_pairs) dalloc(&B.excl_pairs, epairs.size(), owned_);
+        dalloc(&B.excl_off, eoff.size(), owned_, B.stream);
+        dalloc(&B.excl_idx, std::max<size_t>(1, eidx.size()), owned_, B.stream);
+        if (B.n_excl_pairs) dalloc(&B.excl_pairs, epairs.size(), owned_, B.stream);
         B.n_spread_tiles = (int)sp_tiles.size();
         B.n_spread_direct_slots = direct_slots;
-        dalloc(&B.spread_tiles, std::max<size_t>(1, sp_tiles.size()), owned_);
+        dalloc(&B.spread_tiles, std::max<size_t>(1, sp_tiles.size()), owned_, B.stream);
         upload(B.spread_tiles, sp_tiles, st);
-        dalloc(&B.fftz_items, zi.size(), owned_);
-        dalloc(&B.ffty_items, yi.size(), owned_);
-        dalloc(&B.fftx_items, xi.size(), owned_);
+        dalloc(&B.fftz_items, zi.size(), owned_, B.stream);
+        dalloc(&B.ffty_items, yi.size(), owned_, B.stream);
+        dalloc(&B.fftx_items, xi.size(), owned_, B.stream);
         B.n_fftz = (int)zi.size();
         B.n_ffty = (int)yi.size();
         B.n_fftx = (int)xi.size();

@@ -12,24 +12,14 @@
 
 namespace nx {
 
-// Slot-order position of atom a for the pair and PME kernels: the raw
-// coordinates (FP64 path) or, in the mixed path, the build frame -- the wrapped
-// build position plus the displacement since the build -- in which the pair
-// list's per-entry images stay valid until the next build.
+// Slot-order position (x, y, z, q) for the pair and PME kernels.
 template <class P>
-__device__ __forceinline__ typename P::real4 slot_position(const SysDev& d, double x, double y, double z, double q,
-                                                           double xb, double yb, double zb, float4 wb) {
+__device__ __forceinline__ typename P::real4 slot_position(double x, double y, double z, double q) {
     using real = typename P::real;
     typename P::real4 p;
-    if constexpr (P::mixed) {
-        p.x = (real)((double)wb.x + min_image1(x - xb, d.L[0]));
-        p.y = (real)((double)wb.y + min_image1(y - yb, d.L[1]));
-        p.z = (real)((double)wb.z + min_image1(z - zb, d.L[2]));
-    } else {
-        p.x = (real)x;
-        p.y = (real)y;
-        p.z = (real)z;
-    }
+    p.x = (real)x;
+    p.y = (real)y;
+    p.z = (real)z;
     p.w = (real)q;
     return p;
 }
@@ -41,9 +31,7 @@ template <class P>
 __global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
                             const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
                             const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
-                            const double* __restrict__ Q, const double* __restrict__ XB,
-                            const double* __restrict__ YB, const double* __restrict__ ZB,
-                            const float4* __restrict__ wpos, typename P::real4* pos, typename P::facc* FX,
+                            const double* __restrict__ Q, typename P::real4* pos, typename P::facc* FX,
                             typename P::facc* FY, typename P::facc* FZ, int n_slots_total) {
     using real = typename P::real;
     const int sl = blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,7 +40,7 @@ __global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __res
     const int a = (sl - sd[s].slot_off < sc[s].n_slots) ? atom_of_slot[sl] : -1;
     typename P::real4 p;
     if (a >= 0) {
-        p = slot_position<P>(sd[s], X[a], Y[a], Z[a], Q[a], XB[a], YB[a], ZB[a], wpos[sl]);
+        p = slot_position<P>(X[a], Y[a], Z[a], Q[a]);
     } else {
         const real nan = __builtin_nan("");
         p.x = p.y = p.z = nan;
@@ -119,7 +107,7 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
                                  const double* __restrict__ YB, const double* __restrict__ ZB,
                                  typename P::facc* FAx, typename P::facc* FAy, typename P::facc* FAz,
                                  typename P::facc* FSx, typename P::facc* FSy, typename P::facc* FSz,
-                                 const float4* __restrict__ wpos, typename P::real4* pos, double dt, int n_atoms) {
+                                 typename P::real4* pos, double dt, int n_atoms) {
     using real = typename P::real;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
@@ -149,7 +137,7 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
             X[a] = x; Y[a] = y; Z[a] = z;
             VX[a] = vx; VY[a] = vy; VZ[a] = vz;
         }
-        pos[sl] = slot_position<P>(d, x, y, z, Q[a], XB[a], YB[a], ZB[a], wpos[sl]);
+        pos[sl] = slot_position<P>(x, y, z, Q[a]);
         const double dx = min_image1(x - XB[a], d.L[0]);
         const double dy = min_image1(y - YB[a], d.L[1]);
         const double dz = min_image1(z - ZB[a], d.L[2]);
@@ -178,8 +166,7 @@ template <class P>
 void launch_pack(Batch<P>& B) {
     const int T = 256;
     pack_kernel<P><<<(unsigned)((B.n_slots_cap + T - 1) / T), T, 0, B.stream>>>(
-        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.X, B.Y, B.Z, B.q, B.XB, B.YB, B.ZB, B.slot_wpos, B.pos, B.FSx,
-        B.FSy, B.FSz, B.n_slots_cap);
+        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.X, B.Y, B.Z, B.q, B.pos, B.FSx, B.FSy, B.FSz, B.n_slots_cap);
 }
 
 template <class P>
@@ -195,7 +182,7 @@ void launch_integrate(Batch<P>& B, double dt) {
     const int T = 256;
     integrate_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
         B.d_sys, B.d_cnt, B.atom_sys, B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.invm, B.q, B.XB, B.YB, B.ZB,
-        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.slot_wpos, B.pos, dt, B.n_atoms);
+        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.pos, dt, B.n_atoms);
 }
 
 template <class P>

@@ -166,11 +166,7 @@ __global__ void nl_scatter(const SysDev* sd, SysCount* sc, const int* atom_sys, 
 
 // One block (256 threads) per column: deterministic z order, cluster boxes,
 // super items, and the slot-order data of the force kernels (pos: x, y, z, q;
-// NaN coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).  The
-// mixed path's positions are in the build frame (the wrapped build position
-// plus the displacement since the build), so that the list's per-entry images
-// hold until the next build; the FP64 path keeps the raw coordinates for the
-// reference's per-pair nearest image.
+// NaN coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).
 template <class P>
 __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc, const int* col_sys,
                                                   const int* col_count, int* col_fill, const int* col_slot,
@@ -243,15 +239,9 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
             wrapped(d, x, y, z, w);
             slot_wpos[slot] = make_float4((float)w[0], (float)w[1], (float)w[2], 1.0f);
             typename P::real4 p;
-            if constexpr (P::mixed) {
-                p.x = (real)(float)w[0];
-                p.y = (real)(float)w[1];
-                p.z = (real)(float)w[2];
-            } else {
-                p.x = (real)x;
-                p.y = (real)y;
-                p.z = (real)z;
-            }
+            p.x = (real)x;
+            p.y = (real)y;
+            p.z = (real)z;
             p.w = (real)Q[a];
             pos[slot] = p;
             typename P::real2 l;
@@ -395,7 +385,6 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
 
     __shared__ unsigned int l_ent[kListCap];      // cj | imask << 24
     __shared__ int l_eid[kListCap];
-    __shared__ unsigned char l_img[kListCap];     // j image code
     __shared__ unsigned int x_rec[kExclCap];      // (entry * 8 + i-cluster) << 6 | pair bit
     __shared__ int n_x, id_base, lbase, cbase;
     __shared__ float4 ic[8], ih[8];
@@ -487,14 +476,7 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     if (ne > kListCap) { overflow = true; ne = kListCap; }
 
     // 3: atom pairs, one wavefront per entry, batches of eight entries per
-    // wavefront so that their j loads overlap; the kept mask and the entry's
-    // image code are written in place.  The code's image S comes from the
-    // cluster centres of the first i cluster; it stands for the entry when every
-    // pair of its kept tiles that is within rlist at build has S as its nearest
-    // image (a pair beyond rlist cannot come within the cutoff before the next
-    // build under any image).  In a box shorter than 2 rlist a pair can have two
-    // images within rlist, so every entry there takes images pair by pair.
-    const bool small_box = fminf(Lx, fminf(Ly, Lz)) < 2.0f * sqrtf(rl2) + 1e-3f;
+    // wavefront so that their j loads overlap; the kept mask is written in place
     {
         const int ia = lane >> 3, ja = lane & 7;
         float4 pi[8];
@@ -502,43 +484,31 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         for (int k = 0; k < 8; ++k) pi[k] = ip[k * 8 + ia];
         for (int e0 = wv * 8; e0 < ne; e0 += kPlThreads / 8) {
             unsigned int ent[8];
-            float4 pj[8], jcen[8];
+            float4 pj[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 ent[u] = (e0 + u < ne) ? l_ent[e0 + u] : 0u;
                 pj[u] = slot_wpos[(ent[u] & 0xffffffu) * 8 + ja];
-                jcen[u] = cl_ctr[ent[u] & 0xffffffu];
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int cj = (int)(ent[u] & 0xffffffu);
                 const unsigned int im = ent[u] >> 24;
-                const int k0 = im ? __builtin_ctz(im) : 0;
-                const float Sx = rintf((ic[k0].x - jcen[u].x) * iLx), Sy = rintf((ic[k0].y - jcen[u].y) * iLy),
-                            Sz = rintf((ic[k0].z - jcen[u].z) * iLz);
                 unsigned int keep = 0;
-                bool per_pair = small_box;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     if (im & (1u << k)) {
-                        const float ux = pi[k].x - pj[u].x, uy = pi[k].y - pj[u].y, uz = pi[k].z - pj[u].z;
-                        const float nx = rintf(ux * iLx), ny = rintf(uy * iLy), nz = rintf(uz * iLz);
-                        const float dx = ux - Lx * nx, dy = uy - Ly * ny, dz = uz - Lz * nz;
+                        float dx = pi[k].x - pj[u].x, dy = pi[k].y - pj[u].y, dz = pi[k].z - pj[u].z;
+                        dx -= Lx * rintf(dx * iLx);
+                        dy -= Ly * rintf(dy * iLy);
+                        dz -= Lz * rintf(dz * iLz);
                         // NaN (padding) compares false
                         bool ok = dx * dx + dy * dy + dz * dz < rl2;
                         if (cj == fc + k) ok = ok && ja > ia;
-                        const bool bad = ok && !(nx == Sx && ny == Sy && nz == Sz);
-                        if (__ballot(ok)) {
-                            keep |= 1u << k;
-                            if (__ballot(bad)) per_pair = true;
-                        }
+                        if (__ballot(ok)) keep |= 1u << k;
                     }
                 }
-                if (lane == 0 && e0 + u < ne) {
-                    l_ent[e0 + u] = (unsigned)cj | (keep << 24);
-                    l_img[e0 + u] = (unsigned char)(per_pair ? kImagePerPair
-                                                            : (unsigned)((Sx + 1.f) * 9.f + (Sy + 1.f) * 3.f + (Sz + 1.f)));
-                }
+                if (lane == 0 && e0 + u < ne) l_ent[e0 + u] = (unsigned)cj | (keep << 24);
             }
         }
     }
@@ -548,14 +518,10 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         for (int b = 0; b < ne; b += kPlThreads) {
             const int e = b + tid;
             const unsigned int v = (e < ne) ? l_ent[e] : 0u;
-            const unsigned char img = (e < ne) ? l_img[e] : 0;
             const bool kept = (v >> 24) != 0u;
             int cnt;
             const int pos = nk + block_excl_scan(kept, wsum, cnt);   // every read precedes every write
-            if (kept) {
-                l_ent[pos] = v;
-                l_img[pos] = img;
-            }
+            if (kept) l_ent[pos] = v;
             nk += cnt;
             __syncthreads();
         }
@@ -643,8 +609,7 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     for (int e = tid; e < ne; e += kPlThreads) {
         uint2 v;
         v.x = l_ent[e];
-        v.y = ((l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : kNoMask) |
-              ((unsigned)l_img[e] << kEidBits);
+        v.y = (l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : 0xffffffffu;
         list[d.list_off + lbase + e] = v;
     }
     if (tid == 0) {

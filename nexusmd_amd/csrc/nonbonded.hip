@@ -210,15 +210,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
             // every lane loaded the same list word: move it to scalar registers so the
             // per-cluster tests below are scalar branches
             const unsigned ex = __builtin_amdgcn_readfirstlane(ent.x);
-            const unsigned ey = __builtin_amdgcn_readfirstlane(ent.y);
-            const unsigned eid = ey & kNoMask, img = ey >> kEidBits;
+            const unsigned eid = __builtin_amdgcn_readfirstlane(ent.y);
             const int cj = (int)(ex & 0xffffffu);
             const unsigned im = ex >> 24;
             // exclusion bits of the 8 cluster pairs, one load before the cluster loop
             unsigned long long em[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) em[k] = 0ull;
-            if (eid != kNoMask) {
+            if (eid != 0xffffffffu) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) em[k] = emask[(size_t)eid * 8 + k];
             }
@@ -232,20 +231,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                 lj_n = lj[sjn];
             }
             real fxj = 0, fyj = 0, fzj = 0;
-            // mixed path: j moved to the entry's image once (build frame), so the
-            // tiles need no nearest-image step unless the entry takes images pair
-            // by pair; the FP64 path keeps the reference's per-pair correction
-            real4 pjs = pj;
-            bool per_pair = true;
-            if constexpr (P::mixed) {
-                if (img != kImagePerPair) {
-                    per_pair = false;
-                    const int c = (int)img;
-                    pjs.x += Lx * (real)(c / 9 - 1);
-                    pjs.y += Ly * (real)((c / 3) % 3 - 1);
-                    pjs.z += Lz * (real)(c % 3 - 1);
-                }
-            }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if (im & (1u << k)) {
@@ -256,19 +241,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     const real2 li = ilj[k * 8 + ia];
                     real dx, dy, dz;
                     if constexpr (P::mixed) {
-                        dx = pi.x - pjs.x;
-                        dy = pi.y - pjs.y;
-                        dz = pi.z - pjs.z;
-                        if (per_pair) {
-                            // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
-                            dx -= Lx * rintf(dx * iLx);
-                            dy -= Ly * rintf(dy * iLy);
-                            dz -= Lz * rintf(dz * iLz);
-                        }
+                        // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
+                        dx = pi.x - pj.x;
+                        dy = pi.y - pj.y;
+                        dz = pi.z - pj.z;
+                        dx -= Lx * rintf(dx * iLx);
+                        dy -= Ly * rintf(dy * iLy);
+                        dz -= Lz * rintf(dz * iLz);
                     } else {
-                        dx = min_image1(pi.x - pjs.x, Lx);
-                        dy = min_image1(pi.y - pjs.y, Ly);
-                        dz = min_image1(pi.z - pjs.z, Lz);
+                        dx = min_image1(pi.x - pj.x, Lx);
+                        dy = min_image1(pi.y - pj.y, Ly);
+                        dz = min_image1(pi.z - pj.z, Lz);
                     }
                     real r2;
                     if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
@@ -277,14 +260,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     real ecv, elv, dEdR;
                     if constexpr (P::mixed) {
                         // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
-                        dEdR = pair_mixed(r2, pi.w * pjs.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on, sw_inv,
+                        dEdR = pair_mixed(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on, sw_inv,
                                           ecv, elv);
                         dEdR = on ? dEdR : 0.0f;
                     } else {
                         dEdR = 0;
                         ecv = elv = 0;
                         if (on)
-                            dEdR = pair_fp64(r2, pi.w * pjs.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
+                            dEdR = pair_fp64(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
                                              sw_inv, ecv, elv);
                     }
                     // off lanes may hold NaN displacements (padding slots): select both factors
@@ -386,8 +369,7 @@ __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, cons
             if (!(im & (1u << k))) continue;
             const int ai = atom_of_slot[(fc + k) * 8 + ia];
             bool excl = (cj == fc + k) && (ja <= ia);
-            if ((ent.y & kNoMask) != kNoMask)
-                excl |= ((emask[(size_t)(ent.y & kNoMask) * 8 + k] >> lane) & 1ull) != 0ull;
+            if (ent.y != 0xffffffffu) excl |= ((emask[(size_t)ent.y * 8 + k] >> lane) & 1ull) != 0ull;
             if (excl || ai < 0 || aj < 0) continue;
             const double dx = min_image1(X[ai] - X[aj], Lx);
             const double dy = min_image1(Y[ai] - Y[aj], Ly);

@@ -73,16 +73,6 @@ constexpr int kPairWaves = 4;           // independent wavefronts per 256-thread
 // it per chunk; the wavefronts' contiguous chunk ranges are cut to equal sums.
 constexpr int kCostTile = 4, kCostEntry = 2, kCostChunk = 16;
 
-// Pair-list entry: x = j cluster | i-cluster mask << 24; y = exclusion-mask id
-// in the low kEidBits (kNoMask: no excluded pair in the entry's tiles) | the
-// j image code << kEidBits.  Image code (sx+1)*9 + (sy+1)*3 + (sz+1): every
-// pair of the entry's tiles that can come within the cutoff before the next
-// build has its nearest image at j + (sx Lx, sy Ly, sz Lz) in the build frame;
-// kImagePerPair: take the nearest image pair by pair.
-constexpr unsigned kEidBits = 27;
-constexpr unsigned kNoMask = (1u << kEidBits) - 1u;
-constexpr unsigned kImagePerPair = 31u;
-
 // ------------------------------------------------------------ precision
 
 struct PrecFP64 {
