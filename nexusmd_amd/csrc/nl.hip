@@ -634,10 +634,13 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
 // chunk g goes to wavefront floor(NW * (midpoint of g's cost interval) / total
 // cost), so wave_start[w] is the first chunk whose midpoint reaches w / NW of
 // the total.
+constexpr int kWavesLdsSys = 1024;     // systems whose chunk prefix nl_waves keeps in LDS
+
 __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost,
                                                  int* prefix, int* wave_start, int NW) {
     __shared__ int any, total;
     __shared__ long long wsum[16];
+    __shared__ int pre_s[kWavesLdsSys + 1], off_s[kWavesLdsSys];
     const int tid = threadIdx.x;
     if (tid == 0) {
         int a = 0;
@@ -646,9 +649,11 @@ __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc,
             int t = 0;
             for (int s = 0; s < nsys; ++s) {
                 prefix[s] = t;
+                if (s < kWavesLdsSys) { pre_s[s] = t; off_s[s] = sd[s].chunk_off; }
                 t += min(sc[s].n_chunks, sd[s].chunk_cap);     // the counter runs past the capacity on overflow
             }
             prefix[nsys] = t;
+            if (nsys <= kWavesLdsSys) pre_s[nsys] = t;
             total = t;
         }
         any = a;
@@ -656,17 +661,19 @@ __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc,
     __syncthreads();
     if (!any) return;
     const int T = total;
+    const bool lds = nsys <= kWavesLdsSys;
+    auto pre = [&](int s) { return lds ? pre_s[s] : prefix[s]; };
     auto sys_of = [&](int g) {          // largest s with prefix[s] <= g
         int lo = 0, hi = nsys - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
+            if (pre(mid) <= g) lo = mid; else hi = mid - 1;
         }
         return lo;
     };
     auto cost_of = [&](int g, int& s) {
-        while (g >= prefix[s + 1]) ++s;
-        return (long long)chunk_cost[sd[s].chunk_off + g - prefix[s]];
+        while (g >= pre(s + 1)) ++s;
+        return (long long)chunk_cost[(lds ? off_s[s] : sd[s].chunk_off) + g - pre(s)];
     };
     const int per = (T + (int)blockDim.x - 1) / (int)blockDim.x;
     const int g_begin = min(T, tid * per), g_end = min(T, g_begin + per);

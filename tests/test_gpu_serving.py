@@ -81,6 +81,20 @@ def test_fp64_alone_vs_mix_within_tolerance(requests):
         assert abs(e["potential"] - ea["potential"]) <= 1e-6 * abs(ea["potential"])
 
 
+def test_fp64_repeated_runs_agree(requests):
+    """A request run again and again, with other servers allocating and freeing
+    device memory in between, gives the same FP64 result each time (new device
+    buffers are cleared in stream order before their uploads)."""
+    s, k = requests[3]
+    print("[gpu] serving fp64 repeats: n=%d steps=%d" % (s.n, k), flush=True)
+    p0, v0, e0 = _alone(s, k, L.FP64)
+    for _ in range(8):
+        _mix(requests[:3], L.MIXED)
+        p, v, e = _alone(s, k, L.FP64)
+        assert rel_rms(p, p0) <= 1e-9
+        assert rel_rms(v, v0) <= 1e-5
+
+
 def test_admission_by_memory_budget(requests):
     """With a budget that holds about two requests, the rest wait and all still finish."""
     per = []
