@@ -99,9 +99,13 @@ def test_forces_vs_oracle(systems, name, prec, term, mask):
     ctx.close()
 
 
+@pytest.mark.parametrize("passes", ["plane", "lines"])
 @pytest.mark.parametrize("shape", [(24, 24, 24), (32, 30, 20), (40, 36, 48), (64, 64, 64), (96, 96, 96)])
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
-def test_fft_vs_numpy(shape, prec):
+def test_fft_vs_numpy(shape, prec, passes, monkeypatch):
+    """The plane passes (whole planes in LDS) and the five line passes the engine
+    falls back to where a plane does not fit, against numpy's FFT."""
+    monkeypatch.setenv("NEXUSMD_FFT_PLANE", "1" if passes == "plane" else "0")
     rng = np.random.default_rng(5)
     a = rng.normal(size=shape)
     got = E.fft3d_r2c(a, precision=prec)
@@ -111,6 +115,21 @@ def test_fft_vs_numpy(shape, prec):
     back = E.fft3d_c2r(ref, shape[2], precision=prec)
     refb = np.fft.irfftn(ref, s=shape) * a.size
     assert np.max(np.abs(back - refb)) <= tol * np.max(np.abs(refb))
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_forces_with_line_fft_passes(systems, prec, monkeypatch):
+    """The engine on the five line FFT passes (its path where a plane does not
+    fit in LDS) gives the oracle's forces and reciprocal energy."""
+    monkeypatch.setenv("NEXUSMD_FFT_PLANE", "0")
+    sys = systems["water"]
+    ctx = _ctx(sys, prec, skin=0.1)
+    f, e = ctx.compute_forces(L.ALL)
+    fr, er = O.compute_forces(O.Prepared(sys), mask=L.ALL)
+    assert rel_rms(f, fr) <= (FP64_TOL if prec == L.FP64 else MIXED_TOL)
+    etol = 1e-9 if prec == L.FP64 else 2e-5
+    assert abs(e["recip"] - er["recip"]) <= etol * max(1.0, abs(er["recip"])) + (1e-6 if prec == L.FP64 else 1e-3)
+    ctx.close()
 
 
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
