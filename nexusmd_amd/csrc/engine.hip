@@ -370,6 +370,7 @@ class Engine : public EngineBase {
         B.pair_blocks = pair_grid_blocks<P>(B.n_cu, B.n_chunk_cap);
         B.pair_waves = B.pair_blocks * kPairWaves;
         dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o, B.stream);
+        dalloc(&B.pair_pool, (size_t)kPoolInts, o, B.stream);
         dalloc(&B.list, (size_t)B.n_list_cap, o, B.stream);
         dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o, B.stream);
         dalloc(&B.rgrid, (size_t)B.rgrid_total, o, B.stream);

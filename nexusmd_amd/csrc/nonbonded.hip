@@ -7,8 +7,9 @@
 // NonbondedParamsModifier (:504-513): sigma_ij = sig_i/2 + sig_j/2 and
 // eps'_i = 2 sqrt(eps_i), so E_LJ = eps'_i eps'_j (s^6 - 1) s^6.
 //
-// Work decomposition: one wavefront per super-cluster (up to 8 clusters of 8
-// atoms).  Lane l = 8*ia + ja holds i-atom ia of the current i-cluster and
+// Work decomposition: the list is cut into chunks of up to kChunkEntries
+// entries of one super-cluster (up to 8 clusters of 8 atoms); each wavefront
+// takes a contiguous range of chunks of equal estimated work.  Lane l = 8*ia + ja holds i-atom ia of the current i-cluster and
 // j-atom ja of the current j-cluster, so one (i-cluster, j-cluster) tile is
 // 64 pairs in one VALU pass.  i-forces stay in registers over the whole
 // list; j-forces are reduced across the 8 lanes sharing ja and added once
@@ -119,7 +120,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                                                    const unsigned long long* __restrict__ emask,
                                                    const typename P::real4* __restrict__ pos,
                                                    const typename P::real2* __restrict__ lj, typename P::facc* FX,
-                                                   typename P::facc* FY, typename P::facc* FZ, long long* energy) {
+                                                   typename P::facc* FY, typename P::facc* FZ, long long* energy,
+                                                   int* __restrict__ pool) {
     using real = typename P::real;
     using real4 = typename P::real4;
     using real2 = typename P::real2;
@@ -137,10 +139,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
     real* jf = jf_s[w];
     int* jc = jc_s[w];
 
-    // contiguous range of chunks per wavefront, cut to equal estimated work by
-    // the list build; blocks of one XCD take adjacent ranges
-    const int vw = xcd_remap(blockIdx.x, gridDim.x) * kPairWaves + w;
-    const int g0 = wave_start[vw], g1 = wave_start[vw + 1];
+    // The list build cuts the batch's chunks into one contiguous range of equal
+    // estimated work per wavefront.  The wavefronts of one XCD (blocks b with
+    // b % 8 equal, as the hardware hands them out) pool their ranges and claim
+    // chunks one at a time from an XCD counter until the pool is empty.  A fixed
+    // range per wavefront finished 1.3x apart on one CU: the sequencer issues
+    // for its oldest wavefront first, so the first block placed on a CU ran
+    // ahead of the third.  Adjacent chunks are spatial neighbours, so an XCD's
+    // L2 sees one region.  Every chunk is done whole by one wavefront: its
+    // arithmetic does not depend on who does it or when.
+    const int nb = gridDim.x, xcd = blockIdx.x & 7;
+    const int per = (nb + 7) >> 3, full = (nb & 7) ? (nb & 7) : 8;     // blocks per XCD, as in xcd_remap
+    const int xb = (xcd < full) ? xcd * per : full * per + (xcd - full) * (per - 1);
+    const int xn = (xcd < full) ? per : per - 1;
+    const int lo = wave_start[xb * kPairWaves], hi = wave_start[(xb + xn) * kPairWaves];
+    int* counter = pool + xcd * kPoolStride;
+    // lane 0 claims the next chunk; the result is read (and waited for) later
+    auto claim = [&]() {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        return t;
+    };
+    auto claimed = [&](int t) { return lo + __builtin_amdgcn_readfirstlane(t); };
     const real ke = (real)kOne4PiEps0;
 
     // Everything a chunk needs from memory before its first pair: the chunk
@@ -181,10 +201,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
 
     // The mixed path prefetches the next chunk across the flush below; the FP64
     // kernel is register-bound and fetches at the top of each chunk.
+    int g = claimed(claim());
     Pre cur;
-    if (P::mixed && g0 < g1) cur = fetch(g0);
-    for (int g = g0; g < g1; ++g) {
+    if (P::mixed && g < hi) cur = fetch(g);
+    while (g < hi) {
         if constexpr (!P::mixed) cur = fetch(g);
+        int tk = 0;     // the claim of the chunk after this one
         // chunk values are the same on every lane: keep them in scalar registers
         const int s = __builtin_amdgcn_readfirstlane(cur.s);
         const int fc = __builtin_amdgcn_readfirstlane(cur.ch.x), ncl = __builtin_amdgcn_readfirstlane(cur.ch.y);
@@ -234,15 +256,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                 pj_n = pos[sjn];
                 lj_n = lj[sjn];
             }
+            // claim the next chunk two entries before this chunk's end, after the
+            // loads above: the waits for them then do not wait for the atomic (the
+            // vector memory counter drains in issue order), and its round trip
+            // overlaps the last entries' pairs
+            if (e == max(e0, e1 - 2)) tk = claim();
             real fxj = 0, fyj = 0, fzj = 0;
+            // i data of the next cluster is read from LDS one tile ahead of its use,
+            // so the read's latency overlaps the current tile's arithmetic.  The
+            // empty asm makes the LDS index opaque per entry: otherwise the compiler
+            // hoists all 8 clusters' reads out of the entry loop and holds them in 48
+            // more registers, which spills.
+            int io = ia;
+            asm volatile("" : "+v"(io));
+            const real4* ipe = ipos + io;
+            const real2* ile = ilj + io;
+            real4 pi_n = ipe[0];
+            real2 li_n = ile[0];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
+                const real4 pi = pi_n;
+                const real2 li = li_n;
+                if (k + 1 < 8) {
+                    pi_n = ipe[(k + 1) * 8];
+                    li_n = ile[(k + 1) * 8];
+                }
                 if (im & (1u << k)) {
-                    // excluded lanes: listed exclusions, and on the diagonal tile the
-                    // pairs with ja <= ia (self pairs and the other half)
-                    const bool excl = (((em[k] | (cj == fc + k ? kLowerTriangle : 0ull)) >> lane) & 1ull) != 0ull;
-                    const real4 pi = ipos[k * 8 + ia];
-                    const real2 li = ilj[k * 8 + ia];
+                    // excluded lanes, as a wave-wide bit mask: listed exclusions, and on
+                    // the diagonal tile the pairs with ja <= ia (self pairs and the other half)
+                    const unsigned long long xm = em[k] | (cj == fc + k ? kLowerTriangle : 0ull);
                     real dx, dy, dz;
                     if constexpr (P::mixed) {
                         // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
@@ -260,7 +302,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     real r2;
                     if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
                     else r2 = dot_rn(dx, dy, dz);
-                    const bool on = !excl && r2 < rc2;
+                    // lane l is on when bit l of the cutoff ballot is set and bit l of the
+                    // exclusion mask is clear: the mask logic runs in scalar registers
+                    const bool on =
+                        __builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(r2 < rc2) & ~xm);
                     real ecv, elv, dEdR;
                     if constexpr (P::mixed) {
                         // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
@@ -300,8 +345,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
             pj = pj_n;
             lj_j = lj_n;
         }
+        const int gn = claimed(tk);
         if constexpr (P::mixed) {
-            if (g + 1 < g1) cur = fetch(g + 1);
+            if (gn < hi) cur = fetch(gn);
         }
         wave_lds_sync();
         {
@@ -345,6 +391,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
             }
         }
         wave_lds_sync();   // LDS of this wave is reused by its next chunk
+        g = gn;
     }
 }
 
@@ -414,8 +461,9 @@ int pair_grid_blocks(int n_cu, int n_chunk_cap) {
 // separate launch it also gives the kernels forked to the side stream (bonded
 // terms, PME) a head start on the CUs before the pair kernel's persistent grid
 // takes its slots; without it the step measured about 10% slower.
-__global__ void pair_begin_kernel(SysCount* sc, int nsys) {
+__global__ void pair_begin_kernel(SysCount* sc, int nsys, int* pool) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < 8) pool[s * kPoolStride] = 0;     // no chunk of the pair kernel claimed yet
     if (s >= nsys) return;
     if (sc[s].rebuild) {
         sc[s].rebuild = 0;
@@ -425,15 +473,17 @@ __global__ void pair_begin_kernel(SysCount* sc, int nsys) {
 
 template <class P>
 void launch_pair(Batch<P>& B, bool energy) {
-    pair_begin_kernel<<<(B.nsys + 63) / 64, 64, 0, B.stream>>>(B.d_cnt, B.nsys);
+    pair_begin_kernel<<<(B.nsys + 63) / 64, 64, 0, B.stream>>>(B.d_cnt, B.nsys, B.pair_pool);
     if (B.n_chunk_cap == 0) return;
     const unsigned blocks = (unsigned)B.pair_blocks;
     if (energy)
         pair_kernel<P, true><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
-                                                           B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);
+                                                           B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy,
+                                                           B.pair_pool);
     else
         pair_kernel<P, false><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
-                                                            B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy);
+                                                            B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy,
+                                                            B.pair_pool);
 }
 
 template <class P>

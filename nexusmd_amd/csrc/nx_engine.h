@@ -77,7 +77,8 @@ struct Batch {
     int4* chunks = nullptr;         // (first cluster, n clusters, first entry, end entry)
     int* chunk_prefix = nullptr;    // [nsys + 1] running total of chunks over systems
     int* chunk_cost = nullptr;      // per chunk: estimated pair-kernel work (list build)
-    int* wave_start = nullptr;      // [pair_waves + 1] first chunk of each pair-kernel wavefront
+    int* wave_start = nullptr;      // [pair_waves + 1] chunk ranges of equal estimated work (per XCD: its waves' union)
+    int* pair_pool = nullptr;       // [kPoolInts] per XCD: chunks claimed so far by the running pair kernel
     int pair_blocks = 0, pair_waves = 0;
     int n_cu = 0;                   // compute units of the device
     int n_chunk_cap = 0;
