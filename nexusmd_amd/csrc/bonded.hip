@@ -141,6 +141,7 @@ __global__ void bonded_kernel(const SysDev* __restrict__ sd, const int* __restri
                               const double* __restrict__ X, const double* __restrict__ Y,
                               const double* __restrict__ Z, typename P::real* tF, int n_tslots, int mask,
                               long long* energy) {
+    side_priority();
     using real = typename P::real;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int s = 0, kind = -1;
@@ -198,6 +199,7 @@ __global__ void atom_force_kernel(const SysDev* __restrict__ sd, const int* __re
                                   const double* __restrict__ Z, const double* __restrict__ Q, typename P::facc* FX,
                                   typename P::facc* FY, typename P::facc* FZ, long long* energy, int n_atoms,
                                   int mask) {
+    side_priority();
     using real = typename P::real;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     double eacc = 0.0;

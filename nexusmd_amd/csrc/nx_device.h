@@ -77,6 +77,12 @@ constexpr int kCostTile = 4, kCostEntry = 11, kCostChunk = 24;
 // The pair kernel's chunk counters, one per XCD, 128 bytes apart.
 constexpr int kPoolStride = 32, kPoolInts = 8 * kPoolStride;
 
+// Issue priority of a wavefront of the kernels that run beside the pair kernel
+// (PME, bonded terms), above the pair kernel's default 0: the sequencer
+// otherwise issues for the oldest wavefront first, and the pair kernel's
+// persistent wavefronts are older than every kernel launched after it.
+__device__ __forceinline__ void side_priority() { __builtin_amdgcn_s_setprio(2); }
+
 // ------------------------------------------------------------ precision
 
 struct PrecFP64 {

@@ -88,6 +88,7 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
                                   const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
                                   const typename P::real4* __restrict__ pos, typename P::facc* rgrid,
                                   int n_slots_total) {
+    side_priority();
     using real = typename P::real;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = (int)(t / 5), ix = (int)(t % 5);
@@ -141,6 +142,7 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
                                                         const float* __restrict__ slot_zkey,
                                                         const typename P::real4* __restrict__ pos,
                                                         typename P::facc* rgrid) {
+    side_priority();
     using real = typename P::real;
     using acc = typename P::facc;
     __shared__ acc g[kSpreadLdsPoints];
@@ -260,6 +262,7 @@ __global__ void pme_gather_kernel(const SysDev* __restrict__ sd, const SysCount*
                                   const typename P::real4* __restrict__ pos, const typename P::real* __restrict__ phi,
                                   typename P::facc* FX, typename P::facc* FY, typename P::facc* FZ,
                                   int n_slots_total) {
+    side_priority();
     using real = typename P::real;
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= n_slots_total) return;
@@ -437,6 +440,7 @@ template <class P>
 __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, const int4* __restrict__ items,
                                                  const typename P::facc* __restrict__ rgrid,
                                                  typename P::real2* cgrid, const typename P::real2* __restrict__ tw) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -471,6 +475,7 @@ template <class P>
 __global__ __launch_bounds__(256) void fft_z_inv(const SysDev* __restrict__ sd, const int4* __restrict__ items,
                                                  const typename P::real2* __restrict__ cgrid, typename P::real* phi,
                                                  const typename P::real2* __restrict__ tw) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -503,6 +508,7 @@ template <class P>
 __global__ __launch_bounds__(256) void fft_y(const SysDev* __restrict__ sd, const int4* __restrict__ items,
                                              typename P::real2* cgrid, const typename P::real2* __restrict__ tw,
                                              int TZ, int sign) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -536,6 +542,7 @@ __global__ __launch_bounds__(256) void fft_x_conv(const SysDev* __restrict__ sd,
                                                   typename P::real2* cgrid, const typename P::real* __restrict__ eta,
                                                   const typename P::real2* __restrict__ tw, int TZ,
                                                   long long* energy) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -589,6 +596,7 @@ template <class P>
 __global__ __launch_bounds__(256) void fft_x_plain(const SysDev* __restrict__ sd, const int4* __restrict__ items,
                                                    typename P::real2* cgrid, const typename P::real2* __restrict__ tw,
                                                    int TZ, int sign) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -637,6 +645,7 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_fwd(const SysDev* __r
                                                               const typename P::facc* __restrict__ rgrid,
                                                               typename P::real2* cgrid,
                                                               const typename P::real2* __restrict__ tw) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -689,6 +698,7 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_inv(const SysDev* __r
                                                               const typename P::real2* __restrict__ cgrid,
                                                               typename P::real* phi,
                                                               const typename P::real2* __restrict__ tw) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -745,6 +755,7 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_x_block(const SysDev* __res
                                                             const typename P::real* __restrict__ eta,
                                                             const typename P::real2* __restrict__ tw,
                                                             long long* energy) {
+    side_priority();
     using real = typename P::real;
     using R2 = typename P::real2;
     extern __shared__ unsigned char smem_raw[];
@@ -795,6 +806,7 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_x_block(const SysDev* __res
 
 template <class T>
 __global__ void fill_zero(T* p, long long n) {
+    side_priority();
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = T(0);
 }
