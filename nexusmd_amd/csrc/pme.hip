@@ -64,7 +64,7 @@ __device__ __forceinline__ void grid_index(real x, double invL, int K, int& idx,
     if (!(t >= (real)0)) t = 0;
     int ti = (int)t;
     fr = t - (real)ti;
-    idx = ti % K;
+    idx = ti >= K ? ti - K : ti;     // t is in [0, K]
 }
 
 __device__ __forceinline__ void grid_add(double* g, double v) { unsafeAtomicAdd(g, v); }
@@ -132,6 +132,7 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
 // the region is added to the global grid once, z-contiguous.  An atom outside
 // the region (never, while the rebuild criterion holds) adds directly.
 constexpr int kSpreadLdsPoints = 7680;   // 60 KB of int64 / double
+constexpr int kSpreadTileCols = 144;    // columns per block: at most 12 x 12 (the host's choice)
 
 template <class P>
 __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict__ sd,
@@ -146,6 +147,8 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
     using real = typename P::real;
     using acc = typename P::facc;
     __shared__ acc g[kSpreadLdsPoints];
+    __shared__ int c_first[kSpreadTileCols], c_cnt[kSpreadTileCols], c_a[kSpreadTileCols], c_e[kSpreadTileCols];
+    __shared__ int c_pre[kSpreadTileCols + 1];
     const int4 t = tiles[blockIdx.x];          // (sys, bx, by, bz)
     const int s = t.x;
     const SysDev& d = sd[s];
@@ -180,64 +183,98 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
     if (npts > kSpreadLdsPoints) return;       // host guarantees the fit
     for (int i = threadIdx.x; i < npts; i += blockDim.x) g[i] = acc(0);
     __syncthreads();
-    // atoms of the block's columns in the slab: 5 threads per atom (one per x plane)
-    for (int c = cx0; c < cx1; ++c) {
-        for (int cyy = cy0; cyy < cy1; ++cyy) {
-            const int col = d.col_off + c * d.ncy + cyy;
-            const int first = col_slot[col], cnt = col_count[col];
-            // slots are sorted by build-time z: binary search the slab
-            int a = 0, b = cnt;
-            while (a < b) { const int mid = (a + b) >> 1; if ((double)slot_zkey[first + mid] < z0) a = mid + 1; else b = mid; }
-            int e = a, f = cnt;
-            while (e < f) { const int mid = (e + f) >> 1; if ((double)slot_zkey[first + mid] < z1) e = mid + 1; else f = mid; }
-            const int nat = e - a;
-            for (int w = threadIdx.x; w < nat * 5; w += blockDim.x) {
-                const int slot = first + a + w / 5, ix = w % 5;
-                const typename P::real4 p = pos[slot];
-                int i0, i1, i2;
-                real f0, f1, f2;
-                grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
-                grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
-                grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
-                real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
-                bspline5<real>(f0, tx, dtx);
-                bspline5<real>(f1, ty, dty);
-                bspline5<real>(f2, tz, dtz);
-                int u0 = fx ? i0 : i0 - lo[0];
-                int u1 = fy ? i1 : i1 - lo[1];
-                int u2 = fz ? i2 : i2 - lo[2];
-                if (!fx) { if (u0 < 0) u0 += Kx; else if (u0 >= Kx) u0 -= Kx; }
-                if (!fy) { if (u1 < 0) u1 += Ky; else if (u1 >= Ky) u1 -= Ky; }
-                if (!fz) { if (u2 < 0) u2 += Kz; else if (u2 >= Kz) u2 -= Kz; }
-                const bool inside = (fx || (u0 >= 0 && u0 + 4 < n[0])) && (fy || (u1 >= 0 && u1 + 4 < n[1])) &&
-                                    (fz || (u2 >= 0 && u2 + 4 < n[2]));
-                const real qx = p.w * tx[ix];
-                if (inside) {
-                    int x = u0 + ix;
-                    if (fx && x >= Kx) x -= Kx;
-#pragma unroll
-                    for (int iy = 0; iy < 5; ++iy) {
-                        int y = u1 + iy;
-                        if (fy && y >= Ky) y -= Ky;
-                        const real qxy = qx * ty[iy];
-#pragma unroll
-                        for (int iz = 0; iz < 5; ++iz) {
-                            int z = u2 + iz;
-                            if (fz && z >= Kz) z -= Kz;
-                            grid_add_lds(&g[(x * n[1] + y) * n[2] + z], qxy * tz[iz]);
-                        }
-                    }
-                } else {
-                    const int xi = (i0 + ix) % Kx;
-                    acc* gg = rgrid + d.rgrid_off + (long long)xi * Ky * Kz;
-#pragma unroll
-                    for (int iy = 0; iy < 5; ++iy) {
-                        const int yi = (i1 + iy) % Ky;
-                        const real qxy = qx * ty[iy];
-#pragma unroll
-                        for (int iz = 0; iz < 5; ++iz) grid_add(&gg[(long long)yi * Kz + (i2 + iz) % Kz], qxy * tz[iz]);
-                    }
+    // The block's columns (at most kSpreadTileCols) and, per column, the slots
+    // whose build-time z lies in the slab: the slab bounds are found where the
+    // column's sorted keys step across them, by all threads at once
+    // (independent loads, no per-column search chain).
+    const int ncy_t = cy1 - cy0, ncol = (cx1 - cx0) * ncy_t;
+    for (int q = threadIdx.x; q < ncol; q += blockDim.x) {
+        const int col = d.col_off + (cx0 + q / ncy_t) * d.ncy + cy0 + q % ncy_t;
+        c_first[q] = col_slot[col];
+        c_cnt[q] = col_count[col];
+        c_a[q] = 0;
+        c_e[q] = 0;
+    }
+    __syncthreads();
+    {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int q = wv; q < ncol; q += nw) {
+            const int first = c_first[q], cnt = c_cnt[q];
+            for (int i = lane; i < cnt; i += 64) {
+                const double k1 = (double)slot_zkey[first + i];
+                const double k0 = (i > 0) ? (double)slot_zkey[first + i - 1] : -1.0e300;
+                if (k0 < z0 && z0 <= k1) c_a[q] = i;
+                if (k0 < z1 && z1 <= k1) c_e[q] = i;
+                if (i == cnt - 1) {
+                    if (k1 < z0) c_a[q] = cnt;
+                    if (k1 < z1) c_e[q] = cnt;
                 }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {     // running count of the columns' slab atoms
+        int t = 0;
+        for (int q = 0; q < ncol; ++q) {
+            c_pre[q] = t;
+            t += c_e[q] - c_a[q];
+        }
+        c_pre[ncol] = t;
+    }
+    __syncthreads();
+    const int nat = c_pre[ncol];
+    // 5 threads per atom (one per x plane), over all the block's columns
+    for (int w = threadIdx.x; w < nat * 5; w += blockDim.x) {
+        const int at = w / 5, ix = w - at * 5;
+        int lo_q = 0, hi_q = ncol - 1;          // column q with c_pre[q] <= at < c_pre[q + 1]
+        while (lo_q < hi_q) {
+            const int mid = (lo_q + hi_q + 1) >> 1;
+            if (c_pre[mid] <= at) lo_q = mid; else hi_q = mid - 1;
+        }
+        const int slot = c_first[lo_q] + c_a[lo_q] + (at - c_pre[lo_q]);
+        const typename P::real4 p = pos[slot];
+        int i0, i1, i2;
+        real f0, f1, f2;
+        grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
+        grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
+        grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
+        real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
+        bspline5<real>(f0, tx, dtx);
+        bspline5<real>(f1, ty, dty);
+        bspline5<real>(f2, tz, dtz);
+        int u0 = fx ? i0 : i0 - lo[0];
+        int u1 = fy ? i1 : i1 - lo[1];
+        int u2 = fz ? i2 : i2 - lo[2];
+        if (!fx) { if (u0 < 0) u0 += Kx; else if (u0 >= Kx) u0 -= Kx; }
+        if (!fy) { if (u1 < 0) u1 += Ky; else if (u1 >= Ky) u1 -= Ky; }
+        if (!fz) { if (u2 < 0) u2 += Kz; else if (u2 >= Kz) u2 -= Kz; }
+        const bool inside = (fx || (u0 >= 0 && u0 + 4 < n[0])) && (fy || (u1 >= 0 && u1 + 4 < n[1])) &&
+                            (fz || (u2 >= 0 && u2 + 4 < n[2]));
+        const real qx = p.w * tx[ix];
+        if (inside) {
+            int x = u0 + ix;
+            if (fx && x >= Kx) x -= Kx;
+#pragma unroll
+            for (int iy = 0; iy < 5; ++iy) {
+                int y = u1 + iy;
+                if (fy && y >= Ky) y -= Ky;
+                const real qxy = qx * ty[iy];
+#pragma unroll
+                for (int iz = 0; iz < 5; ++iz) {
+                    int z = u2 + iz;
+                    if (fz && z >= Kz) z -= Kz;
+                    grid_add_lds(&g[(x * n[1] + y) * n[2] + z], qxy * tz[iz]);
+                }
+            }
+        } else {
+            const int xi = (i0 + ix) % Kx;
+            acc* gg = rgrid + d.rgrid_off + (long long)xi * Ky * Kz;
+#pragma unroll
+            for (int iy = 0; iy < 5; ++iy) {
+                const int yi = (i1 + iy) % Ky;
+                const real qxy = qx * ty[iy];
+#pragma unroll
+                for (int iz = 0; iz < 5; ++iz) grid_add(&gg[(long long)yi * Kz + (i2 + iz) % Kz], qxy * tz[iz]);
             }
         }
     }
@@ -283,18 +320,22 @@ __global__ void pme_gather_kernel(const SysDev* __restrict__ sd, const SysCount*
     bspline5<real>(f1, ty, dty);
     bspline5<real>(f2, tz, dtz);
     const real* g = phi + d.rgrid_off;
+    // stencil indices, periodic (i + t < 2K: one conditional subtraction, no division)
+    int zi[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) zi[t] = (i2 + t >= Kz) ? i2 + t - Kz : i2 + t;
     real fx = 0, fy = 0, fz = 0;
 #pragma unroll
     for (int ix = 0; ix < 5; ++ix) {
-        const int xi = (i0 + ix) % Kx;
+        const int xi = (i0 + ix >= Kx) ? i0 + ix - Kx : i0 + ix;
 #pragma unroll
         for (int iy = 0; iy < 5; ++iy) {
-            const int yi = (i1 + iy) % Ky;
+            const int yi = (i1 + iy >= Ky) ? i1 + iy - Ky : i1 + iy;
             const real* row = g + ((long long)xi * Ky + yi) * Kz;
             const real a = dtx[ix] * ty[iy], b = tx[ix] * dty[iy], c = tx[ix] * ty[iy];
 #pragma unroll
             for (int iz = 0; iz < 5; ++iz) {
-                const real v = row[(i2 + iz) % Kz];
+                const real v = row[zi[iz]];
                 fx += a * tz[iz] * v;
                 fy += b * tz[iz] * v;
                 fz += c * dtz[iz] * v;
