@@ -292,47 +292,62 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
     }
 }
 
-// One thread per slot: force from the potential grid (NonbondedForce.cpp:407-461).
+// Force from the potential grid (NonbondedForce.cpp:407-461).  A block of
+// 5 wavefronts covers 64 slots: wavefront ix takes stencil plane i0 + ix of
+// every slot (25 grid loads per lane instead of 125, five times the loads in
+// flight), and the five partial gradients are summed in LDS in plane order.
+constexpr int kGatherSlots = 64;
+
 template <class P>
-__global__ void pme_gather_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
-                                  const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
-                                  const typename P::real4* __restrict__ pos, const typename P::real* __restrict__ phi,
-                                  typename P::facc* FX, typename P::facc* FY, typename P::facc* FZ,
-                                  int n_slots_total) {
+__global__ __launch_bounds__(5 * kGatherSlots) void pme_gather_kernel(const SysDev* __restrict__ sd,
+                                                                     const SysCount* __restrict__ sc,
+                                                                     const int* __restrict__ slot_sys,
+                                                                     const int* __restrict__ atom_of_slot,
+                                                                     const typename P::real4* __restrict__ pos,
+                                                                     const typename P::real* __restrict__ phi,
+                                                                     typename P::facc* FX, typename P::facc* FY,
+                                                                     typename P::facc* FZ, int n_slots_total) {
     side_priority();
     using real = typename P::real;
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= n_slots_total) return;
-    const int s = slot_sys[slot];
-    const SysDev& d = sd[s];
-    if (slot - d.slot_off >= sc[s].n_slots) return;
-    if (atom_of_slot[slot] < 0) return;
-    if (d.grid[0] == 0) return;
-    const typename P::real4 p = pos[slot];
-    const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
-    int i0, i1, i2;
-    real f0, f1, f2;
-    grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
-    grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
-    grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
-    real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
-    bspline5<real>(f0, tx, dtx);
-    bspline5<real>(f1, ty, dty);
-    bspline5<real>(f2, tz, dtz);
-    const real* g = phi + d.rgrid_off;
-    // stencil indices, periodic (i + t < 2K: one conditional subtraction, no division)
-    int zi[5];
-#pragma unroll
-    for (int t = 0; t < 5; ++t) zi[t] = (i2 + t >= Kz) ? i2 + t - Kz : i2 + t;
+    __shared__ real part[3][5][kGatherSlots];
+    const int ix = threadIdx.x / kGatherSlots, l = threadIdx.x - ix * kGatherSlots;
+    const int slot = blockIdx.x * kGatherSlots + l;
+    bool on = slot < n_slots_total;
+    int s = 0;
+    if (on) {
+        s = slot_sys[slot];
+        on = slot - sd[s].slot_off < sc[s].n_slots && atom_of_slot[slot] >= 0 && sd[s].grid[0] != 0;
+    }
     real fx = 0, fy = 0, fz = 0;
+    typename P::real4 p;
+    if (on) {
+        const SysDev& d = sd[s];
+        p = pos[slot];
+        const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
+        int i0, i1, i2;
+        real f0, f1, f2;
+        grid_index<real>(p.x, d.invL[0], Kx, i0, f0);
+        grid_index<real>(p.y, d.invL[1], Ky, i1, f1);
+        grid_index<real>(p.z, d.invL[2], Kz, i2, f2);
+        real tx[5], dtx[5], ty[5], dty[5], tz[5], dtz[5];
+        bspline5<real>(f0, tx, dtx);
+        bspline5<real>(f1, ty, dty);
+        bspline5<real>(f2, tz, dtz);
+        // stencil indices, periodic (i + t < 2K: one conditional subtraction, no division)
+        int zi[5];
 #pragma unroll
-    for (int ix = 0; ix < 5; ++ix) {
+        for (int t = 0; t < 5; ++t) zi[t] = (i2 + t >= Kz) ? i2 + t - Kz : i2 + t;
         const int xi = (i0 + ix >= Kx) ? i0 + ix - Kx : i0 + ix;
+        const real* g = phi + d.rgrid_off;
+        real tix = tx[0], dtix = dtx[0];
+#pragma unroll
+        for (int k = 1; k < 5; ++k)
+            if (ix == k) { tix = tx[k]; dtix = dtx[k]; }
 #pragma unroll
         for (int iy = 0; iy < 5; ++iy) {
             const int yi = (i1 + iy >= Ky) ? i1 + iy - Ky : i1 + iy;
             const real* row = g + ((long long)xi * Ky + yi) * Kz;
-            const real a = dtx[ix] * ty[iy], b = tx[ix] * dty[iy], c = tx[ix] * ty[iy];
+            const real a = dtix * ty[iy], b = tix * dty[iy], c = tix * ty[iy];
 #pragma unroll
             for (int iz = 0; iz < 5; ++iz) {
                 const real v = row[zi[iz]];
@@ -342,10 +357,24 @@ __global__ void pme_gather_kernel(const SysDev* __restrict__ sd, const SysCount*
             }
         }
     }
-    const real q = p.w;
-    facc_add(&FX[slot], -q * (fx * (real)Kx * (real)d.invL[0]));
-    facc_add(&FY[slot], -q * (fy * (real)Ky * (real)d.invL[1]));
-    facc_add(&FZ[slot], -q * (fz * (real)Kz * (real)d.invL[2]));
+    part[0][ix][l] = fx;
+    part[1][ix][l] = fy;
+    part[2][ix][l] = fz;
+    __syncthreads();
+    if (ix == 0 && on) {
+        const SysDev& d = sd[s];
+        real gx = 0, gy = 0, gz = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            gx += part[0][k][l];
+            gy += part[1][k][l];
+            gz += part[2][k][l];
+        }
+        const real q = p.w;
+        facc_add(&FX[slot], -q * (gx * (real)d.grid[0] * (real)d.invL[0]));
+        facc_add(&FY[slot], -q * (gy * (real)d.grid[1] * (real)d.invL[1]));
+        facc_add(&FZ[slot], -q * (gz * (real)d.grid[2] * (real)d.invL[2]));
+    }
 }
 
 // ------------------------------------------------------------------ FFT
@@ -944,9 +973,9 @@ void launch_fft_inverse(Batch<P>& B) {
 template <class P>
 void launch_pme_gather(Batch<P>& B) {
     if (!B.pme_any) return;
-    const int T = 256;
-    pme_gather_kernel<P><<<(unsigned)((B.n_slots_cap + T - 1) / T), T, 0, B.stream>>>(
-        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.pos, B.phi, B.FSx, B.FSy, B.FSz, B.n_slots_cap);
+    pme_gather_kernel<P><<<(unsigned)((B.n_slots_cap + kGatherSlots - 1) / kGatherSlots), 5 * kGatherSlots, 0,
+                           B.stream>>>(B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.pos, B.phi, B.FSx, B.FSy,
+                                       B.FSz, B.n_slots_cap);
 }
 
 template void launch_pme_spread<PrecFP64>(Batch<PrecFP64>&);
