@@ -508,7 +508,7 @@ __device__ __forceinline__ double fixed_to_real(double v, double) { return v; }
 // Z pass forward: R2C.  item = (sys, first line, n lines)
 template <class P>
 __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, const int4* __restrict__ items,
-                                                 const typename P::facc* __restrict__ rgrid,
+                                                 typename P::facc* __restrict__ rgrid,
                                                  typename P::real2* cgrid, const typename P::real2* __restrict__ tw) {
     side_priority();
     using real = typename P::real;
@@ -523,11 +523,13 @@ __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, 
     R2* B = smem + nl * Kz;
     R2* TW = B + nl * Kz;
     load_twiddles(TW, tw + d.tw_off[2], Kz);
-    const typename P::facc* g = rgrid + d.rgrid_off + (long long)line0 * Kz;
+    // the spread grid is cleared as it is read: the next spread adds into it
+    typename P::facc* g = rgrid + d.rgrid_off + (long long)line0 * Kz;
     for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) {
         R2 v;
         v.x = fixed_to_real(g[i], (real)0);
         v.y = 0;
+        g[i] = typename P::facc(0);
         A[i] = v;
     }
     __syncthreads();
@@ -712,7 +714,7 @@ __global__ __launch_bounds__(256) void fft_x_plain(const SysDev* __restrict__ sd
 template <class P>
 __global__ __launch_bounds__(kPlaneThreads) void fft_plane_fwd(const SysDev* __restrict__ sd,
                                                               const int4* __restrict__ items,
-                                                              const typename P::facc* __restrict__ rgrid,
+                                                              typename P::facc* __restrict__ rgrid,
                                                               typename P::real2* cgrid,
                                                               const typename P::real2* __restrict__ tw) {
     side_priority();
@@ -731,13 +733,19 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_fwd(const SysDev* __r
     R2* Bf = A + cap;
     load_twiddles(TWy, tw + d.tw_off[1], Ky);
     load_twiddles(TWz, tw + d.tw_off[2], Kz);
-    const typename P::facc* g = rgrid + d.rgrid_off + (long long)x * Ky * Kz;
+    // the spread grid is cleared as it is read: the next spread adds into it
+    typename P::facc* g = rgrid + d.rgrid_off + (long long)x * Ky * Kz;
     const float inv_kz = 1.0f / (float)Kz;
     for (int i = threadIdx.x; i < np * Kz; i += blockDim.x) {
         const int p = qdiv(i, inv_kz), z = i - p * Kz;
         R2 v;
         v.x = fixed_to_real(g[(2 * p) * Kz + z], (real)0);
-        v.y = (2 * p + 1 < Ky) ? fixed_to_real(g[(2 * p + 1) * Kz + z], (real)0) : (real)0;
+        g[(2 * p) * Kz + z] = typename P::facc(0);
+        v.y = (real)0;
+        if (2 * p + 1 < Ky) {
+            v.y = fixed_to_real(g[(2 * p + 1) * Kz + z], (real)0);
+            g[(2 * p + 1) * Kz + z] = typename P::facc(0);
+        }
         A[i] = v;
     }
     __syncthreads();
@@ -874,18 +882,12 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_x_block(const SysDev* __res
     }
 }
 
-template <class T>
-__global__ void fill_zero(T* p, long long n) {
-    side_priority();
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = T(0);
-}
-
 template <class P>
 void launch_pme_spread(Batch<P>& B) {
     if (!B.pme_any) return;
     const int T = 256;
-    fill_zero<typename P::facc><<<(unsigned)((B.rgrid_total + T - 1) / T), T, 0, B.stream>>>(B.rgrid, B.rgrid_total);
+    // the grid starts cleared (allocation) and the forward transform clears it
+    // again as it reads it, so every spread adds into zeros
     if (B.n_spread_tiles > 0)
         pme_spread_tiled<P><<<B.n_spread_tiles, 256, 0, B.stream>>>(B.d_sys, B.d_cnt, B.spread_tiles, B.col_count,
                                                                     B.col_slot, B.slot_zkey, B.pos, B.rgrid);
