@@ -66,13 +66,14 @@ struct SysCount {
     int n_chunks, pad0, pad1, pad2;
 };
 
-constexpr int kChunkEntries = 32;       // pair-list entries per pair-kernel wavefront
+constexpr int kChunkEntries = 32;       // pair-list entries per chunk, the pair kernel's unit of work
 constexpr int kPairWaves = 4;           // independent wavefronts per 256-thread pair-kernel block
 // Estimated pair-kernel work in quarter tiles: per tile, per list entry (the
 // j loads, the j-force reduction and its LDS staging) and per chunk (i staging
 // and the force flush).  Fitted to per-wavefront times of the 100k-atom step:
 // 0.16 us per tile and 0.46 us per entry.  The list build writes the sum per
-// chunk; the wavefronts' contiguous entry ranges are cut to equal sums.
+// chunk and cuts the chunks into contiguous ranges of equal sums, one per
+// wavefront; the pair kernel pools the ranges of each XCD.
 constexpr int kCostTile = 4, kCostEntry = 11, kCostChunk = 24;
 // The pair kernel's chunk counters, one per XCD, 128 bytes apart.
 constexpr int kPoolStride = 32, kPoolInts = 8 * kPoolStride;
