@@ -8,12 +8,12 @@
 // eps'_i = 2 sqrt(eps_i), so E_LJ = eps'_i eps'_j (s^6 - 1) s^6.
 //
 // Work decomposition: the list is cut into chunks of up to kChunkEntries
-// entries of one super-cluster (up to 8 clusters of 8 atoms); each wavefront
-// takes a contiguous range of chunks of equal estimated work.  Lane l = 8*ia + ja holds i-atom ia of the current i-cluster and
-// j-atom ja of the current j-cluster, so one (i-cluster, j-cluster) tile is
-// 64 pairs in one VALU pass.  i-forces stay in registers over the whole
-// list; j-forces are reduced across the 8 lanes sharing ja and added once
-// per tile column.  The FP64 path tests r^2 < rc^2 with the reference's
+// entries of one super-cluster (up to 8 clusters of 8 atoms); the wavefronts
+// of each XCD claim that XCD's chunks one at a time from a counter.  Lane
+// l = 8*ia + ja holds i-atom ia of the current i-cluster and j-atom ja of the
+// current j-cluster, so one (i-cluster, j-cluster) tile is 64 pairs in one
+// VALU pass.  i-forces stay in registers over the chunk; j-forces are reduced
+// across the 8 lanes sharing ja and added once per entry.  The FP64 path tests r^2 < rc^2 with the reference's
 // minimum image and an uncontracted left-to-right dot product, so its pair
 // set is bit-identical to the reference's.
 #include <algorithm>
