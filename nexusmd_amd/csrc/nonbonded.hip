@@ -304,8 +304,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     else r2 = dot_rn(dx, dy, dz);
                     // lane l is on when bit l of the cutoff ballot is set and bit l of the
                     // exclusion mask is clear: the mask logic runs in scalar registers
-                    const bool on =
-                        __builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(r2 < rc2) & ~xm);
+                    const unsigned long long onm = __builtin_amdgcn_ballot_w64(r2 < rc2) & ~xm;
+                    // a tile with no pair inside the cutoff (kept by the list's outer
+                    // radius) skips the pair arithmetic: a uniform branch
+                    if constexpr (P::mixed) {
+                        if (onm == 0ull) continue;
+                    }
+                    const bool on = __builtin_amdgcn_inverse_ballot_w64(onm);
                     real ecv, elv, dEdR;
                     if constexpr (P::mixed) {
                         // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
