@@ -371,7 +371,6 @@ class Engine : public EngineBase {
         B.pair_waves = B.pair_blocks * kPairWaves;
         dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o, B.stream);
         dalloc(&B.pair_pool, (size_t)kPoolInts, o, B.stream);
-        dalloc(&B.nl_sync, (size_t)kSyncWords, o, B.stream);
         dalloc(&B.list, (size_t)B.n_list_cap, o, B.stream);
         dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o, B.stream);
         dalloc(&B.rgrid, (size_t)B.rgrid_total, o, B.stream);
@@ -652,8 +651,6 @@ class Engine : public EngineBase {
         copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * B.nsys, hipMemcpyDeviceToHost);
         for (auto& v : c) { v.rebuild = 1; v.overflow = 0; }
         copy_sync(B.d_cnt, c.data(), sizeof(SysCount) * B.nsys, hipMemcpyHostToDevice);
-        const int any = 1;
-        copy_sync(B.nl_sync + kSyncAny, &any, sizeof(int), hipMemcpyHostToDevice);
     }
 
     // ---------------------------------------------------------------- steps

@@ -107,7 +107,7 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
                                  const double* __restrict__ YB, const double* __restrict__ ZB,
                                  typename P::facc* FAx, typename P::facc* FAy, typename P::facc* FAz,
                                  typename P::facc* FSx, typename P::facc* FSy, typename P::facc* FSz,
-                                 typename P::real4* pos, int* nl_sync, double dt, int n_atoms) {
+                                 typename P::real4* pos, double dt, int n_atoms) {
     using real = typename P::real;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
@@ -145,10 +145,7 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
         moved = !(d2 <= d.skin_half2);
     }
     // lanes of a wave storing to one flag merge into one write
-    if (moved) {
-        sc[s].rebuild = 1;
-        nl_sync[kSyncAny] = 1;
-    }
+    if (moved) sc[s].rebuild = 1;
 }
 
 template <class P>
@@ -185,7 +182,7 @@ void launch_integrate(Batch<P>& B, double dt) {
     const int T = 256;
     integrate_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
         B.d_sys, B.d_cnt, B.atom_sys, B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.invm, B.q, B.XB, B.YB, B.ZB,
-        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.pos, B.nl_sync, dt, B.n_atoms);
+        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.pos, dt, B.n_atoms);
 }
 
 template <class P>

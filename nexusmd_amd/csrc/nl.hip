@@ -4,8 +4,8 @@
 // NeighborList.cpp:12-154), which rebuilds a std::map voxel hash every step.
 // Here, per rebuild:
 //   1. nl_bin      : counting sort of atoms into xy columns (one atomic per
-//                    distinct column in a wavefront); its last block then
-//   2. (scan)      : per-system exclusive scan -> slot ranges (columns padded
+//                    distinct column in a wavefront);
+//   2. nl_scan     : per-system exclusive scan -> slot ranges (columns padded
 //                    to 8-atom clusters) and super-cluster ranges; clears the
 //                    bin counters for the next build;
 //   3. nl_scatter  : atoms into their column's slot range;
@@ -19,21 +19,15 @@
 //                    rlist of an i-cluster (half list: j >= i), then per tile
 //                    the 64 atom pairs, keeping only tiles with a pair within
 //                    rlist; an 8-bit mask of interacting i-clusters per entry,
-//                    and 64-bit exclusion masks for tiles holding excluded pairs;
-//                    its last block then cuts the chunk ranges of the pair
-//                    kernel's wavefronts.
+//                    and 64-bit exclusion masks for tiles holding excluded pairs.
 // The pair kernel applies r^2 < rc^2 exactly per pair, so the evaluated pair
 // set is the reference's set whatever the buffer (skin) is.
 //
-// Every kernel checks the system's rebuild flag, which the integrator raises
-// when an atom has moved more than skin/2 since the last build; nl_bin and
-// nl_pairlist first check the batch-wide flag raised with it, and leave at
-// once on the steps without a build (their last-block work would otherwise
-// take a ticket in every block).  The kernel launched before the pair kernel
-// clears the flags (nl_clear_flag when no pair kernel follows).  Folding the
-// two single-block steps into the last blocks of the kernels before them
-// saves two launches per step, which cost about 2.5 us each inside a step
-// graph even when the kernels do nothing.  Between builds the integrator
+// Every kernel first checks the system's rebuild flag, which the integrator
+// raises when an atom has moved more than skin/2 since the last build; the
+// last kernel (nl_waves) cuts the chunk ranges of the pair kernel's
+// wavefronts, and the kernel launched before the pair kernel clears the flag
+// (nl_clear_flag when no pair kernel follows).  Between builds the integrator
 // keeps the slot-order positions current.
 #include "nx_device.h"
 #include "nx_engine.h"
@@ -83,31 +77,35 @@ __device__ __forceinline__ int wave_keyed_increment(int* counter, int key) {
     return res;
 }
 
-// True in every thread of the block that finishes last among the grid's
-// blocks (each block calls it once).  Every thread's global writes are made
-// visible device-wide before its block takes a ticket; the last block sees all
-// of them and resets the ticket for the next launch.
-__device__ bool last_block(int* ticket) {
-    __shared__ int last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    if (threadIdx.x == 0) *ticket = 0;
-    return true;
+// Column of every atom and the column counts (into col_tmp, zero between builds).
+__global__ void nl_bin(const SysDev* sd, SysCount* sc, const int* atom_sys, const double* X, const double* Y,
+                       const double* Z, int* atom_col, int* col_tmp, int n_atoms_total) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_atoms_total) return;
+    int s = atom_sys[a];
+    if (!sc[s].rebuild) return;
+    const SysDev& d = sd[s];
+    double w[3];
+    wrapped(d, X[a], Y[a], Z[a], w);
+    int cx = (int)(w[0] / d.colw[0]);
+    int cy = (int)(w[1] / d.colw[1]);
+    cx = cx < 0 ? 0 : (cx >= d.ncx ? d.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= d.ncy ? d.ncy - 1 : cy);
+    int col = d.col_off + cx * d.ncy + cy;
+    atom_col[a] = col;
+    wave_keyed_increment(col_tmp, col);
 }
 
-// System s: scan of the padded column counts; col_tmp -> col_count and back
-// to zero.  All threads of the block call it.
-__device__ void scan_columns(const SysDev* sd, SysCount* sc, int s, int* col_tmp, int* col_count, int* col_slot,
-                             int* col_super) {
+// One block per system: scan padded column counts; col_tmp -> col_count and
+// back to zero.
+__global__ void nl_scan(const SysDev* sd, SysCount* sc, int* col_tmp, int* col_count, int* col_slot,
+                        int* col_super) {
+    int s = blockIdx.x;
     if (!sc[s].rebuild) return;
     const SysDev& d = sd[s];
     const int nc = d.ncx * d.ncy;
     __shared__ int tot_slot, tot_sup, carry_slot, carry_sup;
-    __shared__ int ws[256], wsup[256];
+    __shared__ int ws[1024], wsup[1024];
     if (threadIdx.x == 0) { carry_slot = 0; carry_sup = 0; }
     __syncthreads();
     for (int base = 0; base < nc; base += blockDim.x) {
@@ -152,35 +150,6 @@ __device__ void scan_columns(const SysDev* sd, SysCount* sc, int s, int* col_tmp
         sc[s].n_chunks = 0;
         if (ovf) sc[s].overflow |= 1;
     }
-    __syncthreads();
-}
-
-// Column of every atom and the column counts (into col_tmp, zero between
-// builds); the last block then scans the counts of every rebuilding system.
-// When no system of the batch rebuilds, every block leaves at once.
-__global__ __launch_bounds__(256) void nl_bin(const SysDev* sd, SysCount* sc, const int* atom_sys, const double* X,
-                                              const double* Y, const double* Z, int* atom_col, int* col_tmp,
-                                              int* col_count, int* col_slot, int* col_super, int nsys, int* nl_sync,
-                                              int n_atoms_total) {
-    if (!nl_sync[kSyncAny]) return;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < n_atoms_total) {
-        const int s = atom_sys[a];
-        if (sc[s].rebuild) {
-            const SysDev& d = sd[s];
-            double w[3];
-            wrapped(d, X[a], Y[a], Z[a], w);
-            int cx = (int)(w[0] / d.colw[0]);
-            int cy = (int)(w[1] / d.colw[1]);
-            cx = cx < 0 ? 0 : (cx >= d.ncx ? d.ncx - 1 : cx);
-            cy = cy < 0 ? 0 : (cy >= d.ncy ? d.ncy - 1 : cy);
-            int col = d.col_off + cx * d.ncy + cy;
-            atom_col[a] = col;
-            wave_keyed_increment(col_tmp, col);
-        }
-    }
-    if (!last_block(&nl_sync[kSyncBin])) return;
-    for (int s = 0; s < nsys; ++s) scan_columns(sd, sc, s, col_tmp, col_count, col_slot, col_super);
 }
 
 __global__ void nl_scatter(const SysDev* sd, SysCount* sc, const int* atom_sys, const int* atom_col,
@@ -393,11 +362,14 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
 //      rlist (both atoms move at most skin/2 before the next build, so every
 //      pair that can come within rc is in a kept tile); ordered compaction;
 //   4. exclusion masks for the kept tiles.
-__device__ void pairlist_item(const SysDev* sd, SysCount* sc, const int* super_sys, int4* super_item,
-                              const int* col_count, const int* col_slot, const float4* cl_ctr,
-                              const float4* cl_half, const float4* slot_wpos, const int* atom_of_slot,
-                              const int* slot_of_atom, const int* excl_off, const int* excl_idx, uint2* list,
-                              unsigned long long* emask, int4* chunks, int* chunk_cost, int n_items_total) {
+__global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysCount* sc, const int* super_sys,
+                                                          int4* super_item, const int* col_count,
+                                                          const int* col_slot, const float4* cl_ctr,
+                                                          const float4* cl_half, const float4* slot_wpos,
+                                                          const int* atom_of_slot, const int* slot_of_atom,
+                                                          const int* excl_off, const int* excl_idx, uint2* list,
+                                                          unsigned long long* emask, int4* chunks, int* chunk_cost,
+                                                          int n_items_total) {
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= n_items_total) return;
     const int s = super_sys[item];
@@ -656,17 +628,16 @@ __device__ void pairlist_item(const SysDev* sd, SysCount* sc, const int* super_s
     }
 }
 
-// Last step of the list work, in one block (the last block of nl_pairlist).
-// After a build in any system of the batch: the batch's chunk order (system by
-// system, prefix[]) and, for the NW
+// Last kernel of the list work, one block.  After a build in any system of the
+// batch: the batch's chunk order (system by system, prefix[]) and, for the NW
 // wavefronts of the pair kernel, contiguous chunk ranges of near-equal cost:
 // chunk g goes to wavefront floor(NW * (midpoint of g's cost interval) / total
 // cost), so wave_start[w] is the first chunk whose midpoint reaches w / NW of
 // the total.
-constexpr int kWavesLdsSys = 1024;     // systems whose chunk prefix chunk_ranges keeps in LDS
+constexpr int kWavesLdsSys = 1024;     // systems whose chunk prefix nl_waves keeps in LDS
 
-__device__ void chunk_ranges(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost, int* prefix,
-                             int* wave_start, int NW) {
+__global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost,
+                                                 int* prefix, int* wave_start, int NW) {
     __shared__ int any, total;
     __shared__ long long wsum[16];
     __shared__ int pre_s[kWavesLdsSys + 1], off_s[kWavesLdsSys];
@@ -750,30 +721,10 @@ __device__ void chunk_ranges(const SysDev* sd, SysCount* sc, int nsys, const int
     }
 }
 
-// One block per super-cluster (pairlist_item); the last block to finish then
-// cuts the chunk ranges of the pair kernel (chunk_ranges).  When no system of
-// the batch rebuilds, every block leaves at once.
-__global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysCount* sc, const int* super_sys,
-                                                          int4* super_item, const int* col_count,
-                                                          const int* col_slot, const float4* cl_ctr,
-                                                          const float4* cl_half, const float4* slot_wpos,
-                                                          const int* atom_of_slot, const int* slot_of_atom,
-                                                          const int* excl_off, const int* excl_idx, uint2* list,
-                                                          unsigned long long* emask, int4* chunks, int* chunk_cost,
-                                                          int n_items_total, int nsys, int* prefix,
-                                                          int* wave_start, int NW, int* nl_sync) {
-    if (!nl_sync[kSyncAny]) return;
-    pairlist_item(sd, sc, super_sys, super_item, col_count, col_slot, cl_ctr, cl_half, slot_wpos, atom_of_slot,
-                  slot_of_atom, excl_off, excl_idx, list, emask, chunks, chunk_cost, n_items_total);
-    if (!last_block(&nl_sync[kSyncList])) return;
-    chunk_ranges(sd, sc, nsys, chunk_cost, prefix, wave_start, NW);
-}
-
 // Clears the rebuild flags when no pair kernel follows (its first launch does
 // it otherwise) and counts the builds.
-__global__ void nl_clear_flag(SysCount* sc, int nsys, int* nl_sync) {
+__global__ void nl_clear_flag(SysCount* sc, int nsys) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) nl_sync[kSyncAny] = 0;
     if (s >= nsys) return;
     if (sc[s].rebuild) {
         sc[s].rebuild = 0;
@@ -787,7 +738,8 @@ void launch_nlist(Batch<P>& B, bool clear_flags) {
     hipStream_t st = B.stream;
     auto nb = [](long n, int t) { return (unsigned)((n + t - 1) / t); };
     nl_bin<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.atom_col, B.col_tmp,
-                                          B.col_count, B.col_slot, B.col_super, B.nsys, B.nl_sync, B.n_atoms);
+                                          B.n_atoms);
+    nl_scan<<<B.nsys, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.col_tmp, B.col_count, B.col_slot, B.col_super);
     nl_scatter<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.atom_col, B.col_slot, B.col_fill,
                                               B.slot_tmp, B.n_atoms);
     nl_columns<P><<<B.n_cols, 256, 0, st>>>(B.d_sys, B.d_cnt, B.col_sys, B.col_count, B.col_fill, B.col_slot,
@@ -796,9 +748,9 @@ void launch_nlist(Batch<P>& B, bool clear_flags) {
                                             B.slot_zkey, B.slot_wpos, B.pos, B.lj, B.XB, B.YB, B.ZB);
     nl_pairlist<<<B.n_super_cap, kPlThreads, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
                                               B.cl_ctr, B.cl_half, B.slot_wpos, B.atom_of_slot, B.slot_of_atom, B.excl_off,
-                                              B.excl_idx, B.list, B.emask, B.chunks, B.chunk_cost, B.n_super_cap,
-                                              B.nsys, B.chunk_prefix, B.wave_start, B.pair_waves, B.nl_sync);
-    if (clear_flags) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys, B.nl_sync);
+                                              B.excl_idx, B.list, B.emask, B.chunks, B.chunk_cost, B.n_super_cap);
+    nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves);
+    if (clear_flags) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
 }
 
 template void launch_nlist<PrecFP64>(Batch<PrecFP64>&, bool);

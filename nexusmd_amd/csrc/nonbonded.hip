@@ -466,10 +466,9 @@ int pair_grid_blocks(int n_cu, int n_chunk_cap) {
 // separate launch it also gives the kernels forked to the side stream (bonded
 // terms, PME) a head start on the CUs before the pair kernel's persistent grid
 // takes its slots; without it the step measured about 10% slower.
-__global__ void pair_begin_kernel(SysCount* sc, int nsys, int* pool, int* nl_sync) {
+__global__ void pair_begin_kernel(SysCount* sc, int nsys, int* pool) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < 8) pool[s * kPoolStride] = 0;     // no chunk of the pair kernel claimed yet
-    if (s == 0) nl_sync[kSyncAny] = 0;
     if (s >= nsys) return;
     if (sc[s].rebuild) {
         sc[s].rebuild = 0;
@@ -479,7 +478,7 @@ __global__ void pair_begin_kernel(SysCount* sc, int nsys, int* pool, int* nl_syn
 
 template <class P>
 void launch_pair(Batch<P>& B, bool energy) {
-    pair_begin_kernel<<<(B.nsys + 63) / 64, 64, 0, B.stream>>>(B.d_cnt, B.nsys, B.pair_pool, B.nl_sync);
+    pair_begin_kernel<<<(B.nsys + 63) / 64, 64, 0, B.stream>>>(B.d_cnt, B.nsys, B.pair_pool);
     if (B.n_chunk_cap == 0) return;
     const unsigned blocks = (unsigned)B.pair_blocks;
     if (energy)

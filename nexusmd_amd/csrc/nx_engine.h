@@ -18,11 +18,6 @@ namespace nx {
 struct HostSystem;   // engine.hip
 
 // Plane FFT passes: threads per block, LDS available to one workgroup on gfx950.
-// Words of Batch::nl_sync: set when any system of the batch is due a list
-// build (by the update kernel and by the host, beside the per-system flag;
-// cleared with those flags), and the tickets of the list kernels whose last
-// block finishes the build step that needs every block's results.
-constexpr int kSyncAny = 0, kSyncBin = 1, kSyncList = 2, kSyncWords = 4;
 constexpr int kPlaneThreads = 1024;
 constexpr int kXPassThreads = 512;
 constexpr size_t kMaxBlockLds = 160 * 1024 - 1024;   // less the kernels' static LDS
@@ -84,7 +79,6 @@ struct Batch {
     int* chunk_cost = nullptr;      // per chunk: estimated pair-kernel work (list build)
     int* wave_start = nullptr;      // [pair_waves + 1] chunk ranges of equal estimated work (per XCD: its waves' union)
     int* pair_pool = nullptr;       // [kPoolInts] per XCD: chunks claimed so far by the running pair kernel
-    int* nl_sync = nullptr;         // [kSyncWords] batch-wide rebuild flag and the list kernels' block tickets
     int pair_blocks = 0, pair_waves = 0;
     int n_cu = 0;                   // compute units of the device
     int n_chunk_cap = 0;
