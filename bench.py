@@ -88,7 +88,7 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
 def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
     """Warm-up, then the timed region: exactly `steps` steps replayed as step
     graphs, every 32nd of them as graphs around the pair kernel with HIP events
-    on the context's stream bracketing it, and an event closing every step
+    on the context's stream bracketing it, and an event closing every 32 steps
     (nx_step returns after the stream has drained, so the wall clock around it
     covers exactly the timed steps).  Before it, the same count as
     single-graph replays without events; after it, a shorter run with events

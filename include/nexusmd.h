@@ -144,7 +144,8 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
  *                         graphs around the pair kernel (list before it, terms
  *                         and PME beside it, update after it) with events on
  *                         the main stream around the pair kernel; an event
- *                         closes every step: "pair" (sampled steps) and "step"
+ *                         closes every block of 32 steps and the last step:
+ *                         "pair" (sampled steps) and "step" (per step)
  * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
  * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);
  * NEXUSMD_OVERLAP=0 keeps every kernel on one stream.

@@ -655,7 +655,8 @@ class Engine : public EngineBase {
 
     // ---------------------------------------------------------------- steps
 
-    struct Ev { const char* name; hipEvent_t a, b; };
+    // a..b spans n steps (n = 1 for a single kernel family)
+    struct Ev { const char* name; hipEvent_t a, b; int n = 1; };
     hipEvent_t ev() {
         if (ev_used_ < evpool_.size()) return evpool_[ev_used_++];
         hipEvent_t e;
@@ -682,7 +683,7 @@ class Engine : public EngineBase {
             HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
             auto& t = times_[e.name];
             t.first += ms;
-            t.second += 1;
+            t.second += e.n;
         }
         evs_.clear();
         ev_used_ = 0;
@@ -819,14 +820,16 @@ class Engine : public EngineBase {
             // PME; on the side stream beside the pair kernel), the pair kernel,
             // [update] -- with events on the main stream around the pair kernel;
             // the other steps replay the whole step as one graph.  An event on the
-            // main stream closes every step.
+            // main stream closes every block of kPairSampleEvery steps (and the
+            // last step); a marker after every step would add its own latency to
+            // every step.
             if (!seg_exec_[0] || graph_dt_seg_ != dt || graph_mask_seg_ != mask) build_segments(dt, mask);
             if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
             const bool side = use_side(mask);
             hipEvent_t e0 = ev();
             HIPCHK(hipEventRecord(e0, B.stream));
+            int n0 = 0;   // first step of the open block
             for (int i = 0; i < nsteps; ++i) {
-                hipEvent_t c = ev();
                 if (i % kPairSampleEvery == 0) {
                     hipEvent_t a = ev(), b = ev();
                     HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
@@ -845,9 +848,13 @@ class Engine : public EngineBase {
                 } else {
                     HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
                 }
-                HIPCHK(hipEventRecord(c, B.stream));
-                evs_.push_back({"step", e0, c});
-                e0 = c;
+                if ((i + 1) % kPairSampleEvery == 0 || i + 1 == nsteps) {
+                    hipEvent_t c = ev();
+                    HIPCHK(hipEventRecord(c, B.stream));
+                    evs_.push_back({"step", e0, c, i + 1 - n0});
+                    e0 = c;
+                    n0 = i + 1;
+                }
             }
             steps_ += nsteps;
         } else if (B.timing) {

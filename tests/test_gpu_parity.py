@@ -199,6 +199,35 @@ def test_graph_steps_match_plain(chain, prec):
 
 
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_timed_graph_steps_match_untimed(chain, prec):
+    """Steps replayed with the pair kernel bracketed by events (the timed mode
+    of the benchmark: every 32nd step split into graphs around the pair
+    kernel) end in the state of plain graph replays (mixed: bitwise); the
+    step events cover every step once and the pair events every 32nd step."""
+    a = _ctx(chain, prec, skin=0.1)
+    b = _ctx(chain, prec, skin=0.1)
+    a.step(5, 0.001)
+    b.step(5, 0.001)
+    b.enable_timing(L.TIMING_GRAPH_PAIR)
+    for nsteps, npair in ((70, 3), (6, 1)):     # the counts cover the last call
+        a.step(nsteps, 0.001)
+        b.step(nsteps, 0.001)
+        ms, n = b.kernel_time("step")
+        pms, pn = b.kernel_time("pair")
+        assert n == nsteps and ms > 0
+        assert pn == npair and pms > 0
+    b.enable_timing(L.TIMING_OFF)
+    pa, va = a.get_state()
+    pb, vb = b.get_state()
+    if prec == L.MIXED:
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    else:
+        assert np.allclose(pa, pb, rtol=0, atol=1e-9)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
 def test_exclusions_outside_the_pair_list(prec):
     """Bonded partners farther apart than the list radius: their exclusion
     correction cannot come from list tiles, so the system falls back to the
