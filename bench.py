@@ -8,7 +8,7 @@ plus the leapfrog update, with the neighbour list rebuilt on the device
 whenever an atom has moved more than half the buffer.
 
 Prints ONE JSON line.  `value` is the mixed-precision ns/day over the timed
-region (step graphs replayed; every 32nd step replays as graphs around the pair
+region (step graphs replayed; every 128th step replays as graphs around the pair
 kernel with HIP events bracketing it); the FP64 path, the force errors of both
 paths against the FP64 CPU oracle on identical positions, a per-kernel
 breakdown, the pair-kernel
@@ -87,8 +87,8 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
 
 def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
     """Warm-up, then the timed region: exactly `steps` steps replayed as step
-    graphs, every 32nd of them as graphs around the pair kernel with HIP events
-    on the context's stream bracketing it, and an event closing every 32 steps
+    graphs, every 128th of them as graphs around the pair kernel with HIP events
+    on the context's stream bracketing it, and an event closing every 128 steps
     (nx_step returns after the stream has drained, so the wall clock around it
     covers exactly the timed steps).  Before it, the same count as
     single-graph replays without events; after it, a shorter run with events
@@ -123,7 +123,7 @@ def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
     ctx.enable_timing(L.TIMING_OFF)
     timed = {"pair_ms_per_launch": pair_ms, "pair_launches": pair_n, "step_ms_events": step_ms,
              "breakdown_steps": nb,
-             "launch": ("step graphs; every 32nd step as graphs around the pair kernel with events on it"
+             "launch": ("step graphs; every 128th step as graphs around the pair kernel with events on it"
                         if graphs else "direct launches, events around every kernel family")}
     return ctx, (t1 - t0), kern, stats, 0.5 * (pairs0 + pairs1), graph_s, timed
 
@@ -293,7 +293,7 @@ def copy_calibration(mib=256, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="solvated_chain_100000")
     ap.add_argument("--skin", type=float, default=0.1)

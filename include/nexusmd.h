@@ -140,11 +140,11 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
  *                         "nlist", "pack", "bonded", "atom_forces", "pair",
  *                         "pme_spread", "fft", "pme_gather", "integrate"
  *   NX_TIMING_GRAPH_PAIR  nx_step replays every step as one graph, except
- *                         every 32nd (the first included), which replays as
+ *                         every 128th (the first included), which replays as
  *                         graphs around the pair kernel (list before it, terms
  *                         and PME beside it, update after it) with events on
  *                         the main stream around the pair kernel; an event
- *                         closes every block of 32 steps and the last step:
+ *                         closes every block of 128 steps and the last step:
  *                         "pair" (sampled steps) and "step" (per step)
  * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
  * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);

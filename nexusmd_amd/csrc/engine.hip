@@ -1138,7 +1138,7 @@ class Engine : public EngineBase {
     int grow_depth_ = 0;
     // timing segments: [list, positions], [terms, PME], [pair], [update]
     enum { SEG_FIRST, SEG_SIDE, SEG_PAIR, SEG_LAST, SEG_COUNT };
-    static constexpr int kPairSampleEvery = 32;
+    static constexpr int kPairSampleEvery = 128;
     hipGraph_t seg_[SEG_COUNT] = {};
     hipGraphExec_t seg_exec_[SEG_COUNT] = {};
     hipStream_t side_ = nullptr;            // PME beside the pair kernel

@@ -201,15 +201,15 @@ def test_graph_steps_match_plain(chain, prec):
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
 def test_timed_graph_steps_match_untimed(chain, prec):
     """Steps replayed with the pair kernel bracketed by events (the timed mode
-    of the benchmark: every 32nd step split into graphs around the pair
+    of the benchmark: every 128th step split into graphs around the pair
     kernel) end in the state of plain graph replays (mixed: bitwise); the
-    step events cover every step once and the pair events every 32nd step."""
+    step events cover every step once and the pair events every 128th step."""
     a = _ctx(chain, prec, skin=0.1)
     b = _ctx(chain, prec, skin=0.1)
     a.step(5, 0.001)
     b.step(5, 0.001)
     b.enable_timing(L.TIMING_GRAPH_PAIR)
-    for nsteps, npair in ((70, 3), (6, 1)):     # the counts cover the last call
+    for nsteps, npair in ((300, 3), (6, 1)):     # the counts cover the last call
         a.step(nsteps, 0.001)
         b.step(nsteps, 0.001)
         ms, n = b.kernel_time("step")
