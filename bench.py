@@ -275,6 +275,23 @@ def profiled_traffic(kernel="pair_kernel<mixed, false>"):
     return None
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_pair_mixed_solvated_chain_100000.md")
+
+
+def profiled_valu_busy(kernel="pair_kernel<mixed, false>"):
+    """Fraction of the kernel's cycles with a VALU instruction issuing
+    (rocprofiler's VALUBusy from SQ_ACTIVE_INST_VALU and GRBM_GUI_ACTIVE), from
+    the committed counter summary of this bench command; None when absent."""
+    try:
+        for line in open(VALU_PROFILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 3 and cells[0] == kernel and cells[1] == "VALU busy":
+                return float(cells[2])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def copy_calibration(mib=256, reps=3):
     """A plain copy kernel of known size: 256 MiB read and 256 MiB written per
     launch, to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -384,6 +401,9 @@ def main():
         "traffic_source": "profiles/kernels_solvated_chain_100000.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                           "scaled by a 256 MiB copy)",
         "peak_without_packed_fp32": PEAK_FP32_VALU / 2,
+        "valu_busy": profiled_valu_busy(),
+        "valu_busy_source": "profiles/valu_pair_mixed_solvated_chain_100000.md (rocprofv3 --pmc "
+                            "SQ_ACTIVE_INST_VALU, GRBM_GUI_ACTIVE)",
         "algorithmic_flop_per_launch": flops,
         "pairs_within_cutoff": pairs,
         "pairs_evaluated": tiles * 64,
