@@ -25,37 +25,59 @@ namespace nx {
 
 // Abramowitz & Stegun 7.1.26: erfc(x) ~ t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2),
 // t = 1 / (1 + p x), |error| <= 1.5e-7 for x >= 0.
-__device__ __forceinline__ float erfc_as(float x, float expmx2) {
-    const float p = 0.3275911f;
+constexpr double kAsP = 0.3275911;
+__device__ __forceinline__ float erfc_as_t(float t, float expmx2) {
     const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
                 a5 = 1.061405429f;
-    float t = __builtin_amdgcn_rcpf(1.0f + p * x);
     return t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5)))) * expmx2;
 }
 
-struct PairOut {
-    float dEdR_f;
+constexpr double kLog2E = 1.4426950408889634;   // log2(e)
+
+// Per-system constants of the mixed pair function, formed once per chunk so
+// that the per-pair work starts from r and r^2 directly.
+struct MixedPair {
+    float ap;       // p alpha: the A&S variable is t = 1 / (1 + p alpha r)
+    float ek;       // -alpha^2 log2(e): exp(-(alpha r)^2) = 2^(ek r^2)
+    float cg;       // 2 alpha / sqrt(pi)
+    float sw_on;    // switch start, < 0 without a switch
+    float sw_inv;   // 1 / (rc - r_switch)
+    float sw_b;     // -r_switch / (rc - r_switch): t = r sw_inv + sw_b
+    float sw_d;     // -30 / (rc - r_switch): dsv/dr = sw_d t^2 (1 - t)^2
 };
 
+__device__ __forceinline__ MixedPair mixed_pair_const(const SysDev& d) {
+    MixedPair c;
+    c.ap = (float)(kAsP * d.alpha);
+    c.ek = (float)(-d.alpha * d.alpha * kLog2E);
+    c.cg = (float)(kTwoOverSqrtPi * d.alpha);
+    c.sw_on = (float)d.sw_on;
+    c.sw_inv = (float)d.sw_inv;
+    c.sw_b = (float)(-d.sw_on * d.sw_inv);
+    c.sw_d = (float)(-30.0 * d.sw_inv);
+    return c;
+}
+
 // Mixed path: FP32 pair math.  Returns dE/dr / r (force = d * value on i).
-__device__ __forceinline__ float pair_mixed(float r2, float qq, float sig, float epsij, float alpha, float sw_on,
-                                            float sw_inv, float& ecoul, float& elj) {
+__device__ __forceinline__ float pair_mixed(float r2, float qq, float sig, float epsij, const MixedPair& c,
+                                            float& ecoul, float& elj) {
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
     const float inv_r2 = inv_r * inv_r;
-    const float ar = alpha * r;
-    const float ex = __expf(-ar * ar);
-    const float erfcv = erfc_as(ar, ex);
+    const float ex = __builtin_amdgcn_exp2f(c.ek * r2);                  // exp(-(alpha r)^2)
+    const float erfcv = erfc_as_t(__builtin_amdgcn_rcpf(fmaf(c.ap, r, 1.0f)), ex);
     const float ce = qq * inv_r * erfcv;
-    const float dc = (ce + qq * (float)kTwoOverSqrtPi * alpha * ex) * inv_r2;
+    const float dc = fmaf(qq * c.cg, ex, ce) * inv_r2;
     const float s2 = sig * sig * inv_r2;
     const float s6 = s2 * s2 * s2;
     float le = epsij * (s6 - 1.0f) * s6;
     float dl = epsij * (12.0f * s6 - 6.0f) * s6 * inv_r2;
-    if (sw_on >= 0.f) {   // uniform per system; t = 0 inside r_switch gives sv = 1, sd = 0
-        const float t = fmaxf(r - sw_on, 0.0f) * sw_inv;
-        const float sv = 1.0f + t * t * t * (-10.0f + t * (15.0f - t * 6.0f));
-        const float sd = t * t * (-30.0f + t * (60.0f - t * 30.0f)) * sw_inv;
+    if (c.sw_on >= 0.f) {   // uniform per system; t = 0 inside r_switch gives sv = 1, sd = 0
+        // sv = 1 - 10 t^3 + 15 t^4 - 6 t^5, dsv/dt = -30 t^2 (1 - t)^2
+        const float t = fmaxf(fmaf(r, c.sw_inv, c.sw_b), 0.0f);
+        const float sv = fmaf(t * t * t, fmaf(t, fmaf(t, -6.0f, 15.0f), -10.0f), 1.0f);
+        const float u = fmaf(-t, t, t);
+        const float sd = c.sw_d * u * u;
         dl = sv * dl - le * sd * inv_r;
         le *= sv;
     }
@@ -216,6 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
         const real rc2 = (real)sd[s].rc2;
         const real alpha = (real)sd[s].alpha;
         const real sw_on = (real)sd[s].sw_on, sw_inv = (real)sd[s].sw_inv;
+        const MixedPair mpc = mixed_pair_const(sd[s]);   // mixed path only
         {
             real4 p = cur.p;
             p.w *= ke;
@@ -314,8 +337,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     real ecv, elv, dEdR;
                     if constexpr (P::mixed) {
                         // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
-                        dEdR = pair_mixed(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on, sw_inv,
-                                          ecv, elv);
+                        dEdR = pair_mixed(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, mpc, ecv, elv);
                         dEdR = on ? dEdR : 0.0f;
                     } else {
                         dEdR = 0;
