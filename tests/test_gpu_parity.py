@@ -199,6 +199,35 @@ def test_graph_steps_match_plain(chain, prec):
 
 
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_switch_forces_vs_oracle(prec):
+    """The LJ switch from 0.6 nm to the 1.0 nm cutoff, on uncharged LJ
+    particles on a jittered 0.55 nm lattice (the switch changes their forces
+    by about 3e-2, far more than the tolerance): nonbonded forces and the LJ
+    energy as the oracle's, which follows NonbondedForce.cpp:530-585 without
+    the carry-over of the previous pair's switch values."""
+    from _util import bare_system
+    rng = np.random.default_rng(41)
+    a, m = 0.55, 7
+    g = np.array([[x, y, z] for x in range(m) for y in range(m) for z in range(m)], float) * a + 0.1
+    box = a * m
+    sysq = bare_system(g + rng.normal(0, 0.04, g.shape), [0, 0, 0], [box, box, box],
+                       sig=rng.uniform(0.28, 0.34, len(g)), eps=rng.uniform(0.2, 1.0, len(g)), grid=(32, 32, 32))
+    rsw = 0.6
+    ctx = _ctx(sysq, prec, skin=0.1, use_switch=1, switch_distance=rsw)
+    f, e = ctx.compute_forces(L.NONBONDED)
+    fr, er = O.compute_forces(O.Prepared(sysq, use_switch=1, switch_distance=rsw), mask=O.NONBONDED)
+    f0, _ = O.compute_forces(O.Prepared(sysq), mask=O.NONBONDED)
+    assert rel_rms(fr, f0) > 1e-2      # the switch changes the forces
+    err = rel_rms(f, fr)
+    print("[gpu] switch prec=%d force err %.3e" % (prec, err), flush=True)
+    assert err <= (FP64_TOL if prec == L.FP64 else MIXED_TOL)
+    etol = 1e-9 if prec == L.FP64 else 2e-5
+    for a, b in (("lj", "lj"), ("coulomb_direct", "coulomb")):
+        assert abs(e[a] - er[b]) <= etol * max(1.0, abs(er[b])) + (1e-6 if prec == L.FP64 else 1e-3), (a, e[a], er[b])
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
 def test_timed_graph_steps_match_untimed(chain, prec):
     """Steps replayed with the pair kernel bracketed by events (the timed mode
     of the benchmark: every 128th step split into graphs around the pair

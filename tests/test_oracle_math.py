@@ -51,15 +51,26 @@ def test_bonded_forces_are_minus_gradient(chain, term, mask):
             assert abs(fd - f[i, d]) <= 1e-4 * max(1.0, abs(f[i, d])), (term, i, d, fd, f[i, d])
 
 
-def test_direct_forces_are_minus_gradient():
+@pytest.mark.parametrize("rsw", [None, 0.7])
+def test_direct_forces_are_minus_gradient(rsw):
+    """With rsw, the LJ switch (NonbondedForce.cpp:530-585) from r = rsw to the
+    cutoff, on uncharged particles so that the forces are LJ alone: the
+    switched forces are still minus the gradient of the switched energy, and
+    the switch changes them."""
     rng = np.random.default_rng(5)
     s = bare_system(rng.uniform(0, 2.2, (40, 3)), [0, 0, 0], [2.2, 2.2, 2.2], q=rng.normal(0, 0.5, 40),
                     sig=rng.uniform(0.25, 0.35, 40), eps=rng.uniform(0.1, 1.0, 40), cutoff=1.0)
     # keep particles apart so the LJ terms stay moderate
     s.pos = np.array([[x, y, z] for x in (0.2, 0.75, 1.3, 1.85) for y in (0.2, 0.75, 1.3, 1.85)
                       for z in (0.2, 1.0, 1.7)])[:40] + rng.normal(0, 0.03, (40, 3))
-    prep = O.Prepared(s, excl=(np.zeros(s.n + 1, np.int32), np.zeros(0, np.int32)))
+    if rsw is not None:
+        s.q = np.zeros(s.n)
+    no_excl = (np.zeros(s.n + 1, np.int32), np.zeros(0, np.int32))
+    prep = O.Prepared(s, excl=no_excl, use_switch=int(rsw is not None), switch_distance=rsw or 0.0)
     f, _ = O.nonbonded_real(prep)
+    if rsw is not None:
+        f0, _ = O.nonbonded_real(O.Prepared(s, excl=no_excl))
+        assert rel_rms(f, f0) > 1e-2
     h = 1e-6
     for i in (0, 7, 19, 33):
         for d in range(3):
