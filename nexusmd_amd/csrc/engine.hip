@@ -1022,17 +1022,21 @@ class Engine : public EngineBase {
         while (true) {
             int2* d = nullptr;
             HIPCHK(hipMalloc(&d, sizeof(int2) * std::max<long long>(cap, 1)));
-            long long n = launch_export_pairs<P>(B, d, cap, d_count_);
-            if (!out) { hipFree(d); return n; }
+            std::unique_ptr<int2, hipError_t (*)(void*)> hold(d, hipFree);   // freed on every path out
+            HIPCHK(hipMemsetAsync(d_count_, 0, sizeof(long long), B.stream));
+            launch_export_pairs<P>(B, d, cap, d_count_);
+            HIPCHK(hipGetLastError());
+            long long n = 0;
+            HIPCHK(hipMemcpyAsync(&n, d_count_, sizeof(long long), hipMemcpyDeviceToHost, B.stream));
+            HIPCHK(hipStreamSynchronize(B.stream));
+            if (!out) return n;
             if (n <= cap) {
                 std::vector<int2> h(n);
                 if (n) copy_sync(h.data(), d, sizeof(int2) * n, hipMemcpyDeviceToHost);
-                hipFree(d);
                 out->resize(2 * n);
                 for (long long i = 0; i < n; ++i) { (*out)[2 * i] = h[i].x; (*out)[2 * i + 1] = h[i].y; }
                 return n;
             }
-            hipFree(d);
             cap = n;
         }
     }

@@ -514,23 +514,18 @@ void launch_pair(Batch<P>& B, bool energy) {
 }
 
 template <class P>
-long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count) {
-    hipMemsetAsync(d_count, 0, sizeof(long long), B.stream);
+void launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count) {
     export_pairs_kernel<P><<<B.n_super_cap, 64, 0, B.stream>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.list,
                                                                B.emask, B.atom_of_slot, B.X, B.Y, B.Z, out, cap,
                                                                reinterpret_cast<unsigned long long*>(d_count),
                                                                B.n_super_cap);
-    long long n = 0;
-    hipMemcpyAsync(&n, d_count, sizeof(long long), hipMemcpyDeviceToHost, B.stream);
-    hipStreamSynchronize(B.stream);
-    return n;
 }
 
 template int pair_grid_blocks<PrecFP64>(int, int);
 template int pair_grid_blocks<PrecMixed>(int, int);
 template void launch_pair<PrecFP64>(Batch<PrecFP64>&, bool);
 template void launch_pair<PrecMixed>(Batch<PrecMixed>&, bool);
-template long long launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);
-template long long launch_export_pairs<PrecMixed>(Batch<PrecMixed>&, int2*, long long, long long*);
+template void launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);
+template void launch_export_pairs<PrecMixed>(Batch<PrecMixed>&, int2*, long long, long long*);
 
 }  // namespace nx

@@ -145,7 +145,8 @@ template <class P> void launch_pme_gather(Batch<P>& B);
 template <class P> void launch_kinetic(Batch<P>& B, double dt);
 template <class P> void launch_integrate(Batch<P>& B, double dt);
 template <class P> void launch_collect_forces(Batch<P>& B, double* FX, double* FY, double* FZ);
-template <class P> long long launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count);
+// Writes up to cap pairs to out and their total count to *d_count, which must be zero before the launch.
+template <class P> void launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count);
 // standalone FFT (verification)
 template <class P> void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool forward,
                                         hipStream_t st);
