@@ -7,14 +7,23 @@ full force evaluation (bonded, real-space pairs, exclusion correction, PME)
 plus the leapfrog update, with the neighbour list rebuilt on the device
 whenever an atom has moved more than half the buffer.
 
-Prints ONE JSON line.  `value` is the mixed-precision ns/day over the timed
-region (step graphs replayed; every 128th step replays as graphs around the pair
-kernel with HIP events bracketing it); the FP64 path, the force errors of both
-paths against the FP64 CPU oracle on identical positions, a per-kernel
-breakdown, the pair-kernel
-roofline, a CPU baseline (the oracle, one core, a bounded sample of the same
-workload) and the batched-serving measurement (256 ligand-in-water requests,
-continuous batching vs one at a time, p50/p99 step latency) ride along.
+Prints ONE JSON line.  `value` is the mixed-precision ns/day over exactly K
+steps (--steps) replayed as step graphs, with nothing else on the stream and
+no events inside; the FP64 path is timed over the same K steps.  Beside it
+ride, all measured outside those K steps:
+  - the force errors of both paths against the FP64 CPU oracle on identical
+    positions;
+  - the pair kernel timed over 64 launches as it runs in a step (PME beside
+    it) and over 32 launches alone, and the roofline figure from the former;
+  - a per-kernel breakdown (every kernel family alone, events around each);
+  - a steady-state comparison of both paths over 2,000 steps;
+  - the total-energy drift of both paths over 5,000 steps, per ns;
+  - a CPU baseline: the reference's own functions, compiled from its sources
+    (numpy's FFT standing where FFTW would), on one pinned core, 10 steps of
+    the same system;
+  - the batched-serving measurement (256 ligand-in-water requests,
+    continuous batching vs one at a time, per-step latency from device
+    events).
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-fp64] [--no-serving]
 """
@@ -37,6 +46,10 @@ PEAK_FP32_VALU = 157.3   # TFLOP/s, MI355X FP32 vector (packed); 78.6 without pa
 PEAK_FP64_VALU = 78.6    # TFLOP/s
 PEAK_HBM = 8000.0        # GB/s
 PAIR_FLOP = 75           # FLOP per within-cutoff pair (FMA = 2, rsqrt/exp/rcp = 1)
+PAIR_SAMPLE_STEPS = 64   # steps whose pair kernel is bracketed by events (beside PME)
+PAIR_ALONE_STEPS = 32    # steps of per-kernel events (every kernel family alone)
+STEADY_STEPS = 2000
+DRIFT_STEPS, DRIFT_EVERY = 5000, 250
 
 
 def log(*a):
@@ -64,6 +77,18 @@ def make_workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def workload_config(name, sysm, skin):
+    cfg = {"workload": name, "atoms": sysm.n, "box_nm": float(sysm.L[0]), "pme_grid": list(sysm.grid),
+           "pme_order": 5, "cutoff_nm": sysm.cutoff, "ewald_tol": sysm.ewald_tol, "dt_fs": sysm.dt * 1e3,
+           "skin_nm": skin, "precision": "mixed"}
+    if name == "solvated_chain_100000":
+        # generator choices the configuration leaves open, stated with the numbers
+        cfg.update({"chain": "free self-avoiding walk from the box centre (no globule)",
+                    "h_core": "water H sig 0.04 nm, eps 0.192 kJ/mol beside charged beads",
+                    "relaxed": "300 steps of GPU steepest descent, then Maxwell-Boltzmann 300 K"})
+    return cfg
+
+
 def kernel_bytes(sysm, prec_bytes, stats, n_excl):
     """Algorithmic bytes per launch of the HBM-bound kernels."""
     N = sysm.n
@@ -73,7 +98,6 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
     r = prec_bytes
     pos = 4 * r                                   # x, y, z, q
     nterms = len(sysm.bond_idx) * 2 + len(sysm.angle_idx) * 3 + len(sysm.torsion_idx) * 4
-    nslots = nterms
     return {
         "pme_spread": N * pos + G * 8,
         "fft": 2 * (G * r + Gc * 2 * r),
@@ -81,73 +105,145 @@ def kernel_bytes(sysm, prec_bytes, stats, n_excl):
         "integrate": N * 128,
         "bonded": (len(sysm.bond_idx) * 24 + len(sysm.angle_idx) * 32 + len(sysm.torsion_idx) * 48)
                   + nterms * (24 + 3 * prec_bytes),
-        "atom_forces": N * (8 + 24) + 2 * n_excl * (4 + 24 + 8) + nslots * (4 + 3 * prec_bytes) + N * 24,
+        "atom_forces": N * (8 + 24) + 2 * n_excl * (4 + 24 + 8) + nterms * (4 + 3 * prec_bytes) + N * 24,
     }
 
 
-def time_path(sysm, prec, steps, warmup, skin, breakdown_steps=200):
-    """Warm-up, then the timed region: exactly `steps` steps replayed as step
-    graphs, every 128th of them as graphs around the pair kernel with HIP events
-    on the context's stream bracketing it, and an event closing every 128 steps
-    (nx_step returns after the stream has drained, so the wall clock around it
-    covers exactly the timed steps).  Before it, the same count as
-    single-graph replays without events; after it, a shorter run with events
-    around every kernel family for the per-kernel breakdown."""
+def time_path(sysm, prec, steps, warmup, skin):
+    """Warm-up, then the timed region: exactly `steps` steps as step-graph
+    replays (nx_step returns after its stream has drained, so the wall clock
+    around it covers exactly those steps).  Then, outside the timed region,
+    PAIR_SAMPLE_STEPS steps with events around the pair kernel as it runs
+    beside PME, and PAIR_ALONE_STEPS steps with events around every kernel
+    family launched alone."""
     from nexusmd_amd import _lib as L
     from nexusmd_amd import engine as E
     graphs = os.environ.get("NEXUSMD_GRAPHS", "1") != "0"
     ctx = E.Context(sysm, precision=prec, skin=skin)
     ctx.step(warmup, sysm.dt)
     t0 = time.perf_counter()
-    ctx.step(steps, sysm.dt)                       # one graph per step, no events
-    graph_s = time.perf_counter() - t0
-    pairs0 = ctx.pair_count()
-    # with graphs off (NEXUSMD_GRAPHS=0) the timed region is direct launches with
-    # events around every kernel family, the pair kernel among them
-    ctx.enable_timing(L.TIMING_GRAPH_PAIR if graphs else L.TIMING_KERNELS)
-    ctx.step(2, sysm.dt)                           # builds the segment graphs outside the timed region
-    t0 = time.perf_counter()
     ctx.step(steps, sysm.dt)                       # timed region
-    t1 = time.perf_counter()
-    pair_ms, pair_n = ctx.kernel_time("pair")
-    step_ms, step_n = ctx.kernel_time("step") if graphs else (1e3 * (t1 - t0) / steps, steps)
+    wall = time.perf_counter() - t0
     stats = ctx.stats()
+    # the pair kernel as it runs in a step: graphs around it, PME beside it on the
+    # side stream, events on the main stream (direct launches with NEXUSMD_GRAPHS=0)
+    pairs0 = ctx.pair_count()
+    ctx.enable_timing(L.TIMING_GRAPH_PAIR)
+    ctx.step(PAIR_SAMPLE_STEPS, sysm.dt)
+    beside_ms, beside_n = ctx.kernel_time("pair")
+    step_ev_ms, step_ev_n = ctx.kernel_time("step")
     pairs1 = ctx.pair_count()
+    # every kernel family alone, one stream, events around each
     ctx.enable_timing(L.TIMING_KERNELS)
-    nb = min(steps, breakdown_steps)
-    ctx.step(nb, sysm.dt)
+    ctx.step(PAIR_ALONE_STEPS, sysm.dt)
     kern = {}
     for name in ("pair", "pme_spread", "fft", "pme_gather", "bonded", "atom_forces", "integrate", "nlist", "pack"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"ms_per_launch": ms, "launches": n}
     ctx.enable_timing(L.TIMING_OFF)
-    timed = {"pair_ms_per_launch": pair_ms, "pair_launches": pair_n, "step_ms_events": step_ms,
-             "breakdown_steps": nb,
-             "launch": ("step graphs; every 128th step as graphs around the pair kernel with events on it"
-                        if graphs else "direct launches, events around every kernel family")}
-    return ctx, (t1 - t0), kern, stats, 0.5 * (pairs0 + pairs1), graph_s, timed
+    pair = {"beside_pme_ms_per_launch": beside_ms, "beside_pme_launches": beside_n,
+            "alone_ms_per_launch": kern["pair"]["ms_per_launch"], "alone_launches": kern["pair"]["launches"],
+            "sampled_step_ms": step_ev_ms if step_ev_n else None,
+            "launch": "step graphs" if graphs else "direct launches (NEXUSMD_GRAPHS=0)"}
+    return ctx, wall, kern, stats, 0.5 * (pairs0 + pairs1), pair
 
 
-def cpu_baseline(sysm, budget_s=20.0):
-    """FP64 oracle, one thread, on a bounded sample of the same workload."""
-    from oracle import pyoracle as O
-    prep = O.Prepared(sysm)
+def steady_state(sysm, prec, skin, steps=STEADY_STEPS):
+    """ns/day of `steps` graph-replayed steps after 100 warm-up steps."""
+    from nexusmd_amd import engine as E
+    ctx = E.Context(sysm, precision=prec, skin=skin)
+    ctx.step(100, sysm.dt)
     t0 = time.perf_counter()
-    O.simulate(prep, 1, sysm.dt)
-    one = time.perf_counter() - t0
-    nsteps = max(1, min(20, int(budget_s / max(one, 1e-3)) - 1))
-    t0 = time.perf_counter()
-    O.simulate(prep, nsteps, sysm.dt)
+    ctx.step(steps, sysm.dt)
     wall = time.perf_counter() - t0
+    rebuilds = ctx.stats()["rebuilds"]
+    ctx.close()
+    return {"steps": steps, "ms_per_step": 1e3 * wall / steps, "ns_per_day": ns_per_day(steps, sysm.dt, wall),
+            "rebuilds": rebuilds}
+
+
+def energy_drift(sysm, prec, skin, steps=DRIFT_STEPS, every=DRIFT_EVERY):
+    """Total energy (kinetic with the reference's half-step shift + physical
+    potential) every `every` steps over `steps` steps; drift = least-squares
+    slope in kJ/mol per ns, also per atom."""
+    from nexusmd_amd import engine as E
+    ctx = E.Context(sysm, precision=prec, skin=skin)
+    t, etot = [], []
+    done = 0
+    while done <= steps:
+        _, _, _, e = ctx.step_report(sysm.dt)
+        t.append(done * sysm.dt * 1e-3)                      # ns
+        etot.append(e["kinetic"] + e["potential"])
+        done += 1
+        if done > steps:
+            break
+        ctx.step(every - 1, sysm.dt)
+        done += every - 1
+    ctx.close()
+    t, etot = np.asarray(t), np.asarray(etot)
+    slope = float(np.polyfit(t, etot, 1)[0])
+    return {"steps": steps, "samples": len(t), "kj_mol_per_ns": slope, "kj_mol_per_ns_per_atom": slope / sysm.n,
+            "std_kj_mol": float(np.std(etot)), "e0_kj_mol": float(etot[0])}
+
+
+def host_info():
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_allowed": affinity}
+
+
+def cpu_baseline(sysm, nsteps=10):
+    """The reference's own functions (compiled from its sources into
+    oracle/_ref/libnxref.so; numpy's unnormalised FFT where FFTW would run),
+    composed in Engine::RunSimulation's order (neighbour list rebuilt and full
+    PME every step), on one core pinned with sched_setaffinity, for `nsteps`
+    steps of the same system after one warm-up step.  Without the compiled
+    reference, the C oracle's restatement in its place (kind "port")."""
+    from oracle import pyoracle as O
+    info = host_info()
+    cpus = sorted(os.sched_getaffinity(0))
+    pin = cpus[0]
+    os.sched_setaffinity(0, {pin})
+    try:
+        if O.ref_available():
+            from oracle import pyref as R
+            kind = "reference"
+            R.simulate(sysm, 1, sysm.dt)
+            t0 = time.perf_counter()
+            R.simulate(sysm, nsteps, sysm.dt)
+            wall = time.perf_counter() - t0
+            what = ("the reference's functions compiled from its sources (oracle/_ref), numpy pocketfft in "
+                    "FFTW's place")
+        else:
+            kind = "port"
+            prep = O.Prepared(sysm)
+            O.simulate(prep, 1, sysm.dt)
+            t0 = time.perf_counter()
+            O.simulate(prep, nsteps, sysm.dt)
+            wall = time.perf_counter() - t0
+            what = "the C oracle's restatement of the reference (oracle/libnxo.so)"
+    finally:
+        os.sched_setaffinity(0, set(cpus))
     return {
         "value": ns_per_day(nsteps, sysm.dt, wall),
         "unit": "ns/day",
         "cores": 1,
-        "kind": "port",
-        "sample": "%d leapfrog steps of the same %d-atom system (FP64, neighbour list rebuilt every step, "
-                  "full PME every step), %.1f s wall after one warm-up step" % (nsteps, sysm.n, wall),
+        "kind": kind,
+        "sample": "%d leapfrog steps of the same %d-atom system after one warm-up step (FP64, neighbour list "
+                  "rebuilt and full PME every step, grid %s), %.1f s wall; %s; pinned to CPU %d"
+                  % (nsteps, sysm.n, "x".join(map(str, sysm.grid)), wall, what, pin),
         "s_per_step": wall / nsteps,
-        "host": platform.processor() or platform.machine(),
+        "host": info,
     }
 
 
@@ -157,7 +253,7 @@ def _make_request(args):
     return S.ligand_in_water(n, sd, steps=steps)
 
 
-def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, workers=16):
+def serving_bench(n_requests, steps_per_request, alone_steps, alone_full_sample=4, seed=20261205, workers=16):
     """Continuous batching of ligand-in-water requests (atom counts lognormal,
     median 2,500, clipped to [600, 8000]) against the same requests run one at
     a time, all in the mixed path.
@@ -166,10 +262,15 @@ def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, wor
     rate measured in a short saturated calibration; the server admits them by
     HBM footprint as they arrive and advances every active system together.
     `aggregate_ns_per_day` = sum of simulated time / wall-clock makespan.
+    Per-step latency: for every request, the interval between the device
+    events that close its successive steps (the batch's step events), with
+    the batch re-layout and admission that precede a step inside the interval
+    of that step.
     One at a time: every request runs alone (its own context, graph replays)
     for `alone_steps` steps after warm-up; its per-step time times the
     requested step count is its service time, and the sum of those is the
-    sequential makespan."""
+    sequential makespan.  A sample of requests is also run alone for the
+    full step count, to check the extrapolation."""
     import multiprocessing as mp
 
     from nexusmd_amd import _lib as L
@@ -197,6 +298,19 @@ def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, wor
     alone_ns_day = 86400.0 * sim_ns / alone_makespan
     log("serving: one at a time %.1f ns/day aggregate (median step %.3f ms)" % (
         alone_ns_day, 1e3 * float(np.median(alone_step_s))))
+    # full-length alone runs of a sample (smallest, median, largest, one more)
+    order = np.argsort(sizes)
+    pick = sorted(set(int(order[i]) for i in (0, len(order) // 2, len(order) - 1, len(order) // 4)))[:alone_full_sample]
+    full = []
+    for i in pick:
+        ctx = E.Context(reqs[i], precision=L.MIXED, skin=0.1)
+        ctx.step(20, dt)
+        t0 = time.perf_counter()
+        ctx.step(steps_per_request, dt)
+        s_full = (time.perf_counter() - t0) / steps_per_request
+        ctx.close()
+        full.append({"atoms": reqs[i].n, "ms_per_step_full": 1e3 * s_full,
+                     "ms_per_step_extrapolated": 1e3 * alone_step_s[i]})
     # service-rate calibration: the first 64 requests together, saturated
     cal = E.Server(precision=L.MIXED, budget_bytes=200 << 30, max_active=n_requests)
     cal_reqs = reqs[: min(64, len(reqs))]
@@ -248,6 +362,9 @@ def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, wor
         "batched_over_one_at_a_time": agg / alone_ns_day,
         "p50_step_ms": st["p50_step_ms"],
         "p99_step_ms": st["p99_step_ms"],
+        "latency_sampling": "per (request, step): interval between the device events closing the request's "
+                            "successive batched steps; a step that follows a batch re-layout (admission or "
+                            "retirement) includes the re-layout",
         "finished": st["finished"],
         "failed": st["failed"],
         "makespan_s": wall,
@@ -255,6 +372,7 @@ def serving_bench(n_requests, steps_per_request, alone_steps, seed=20261205, wor
         "one_at_a_time_makespan_s": alone_makespan,
         "one_at_a_time_sample": "every request alone for %d steps after 20 warm-up steps; its per-step time "
                                 "x %d steps is its service time" % (alone_steps, steps_per_request),
+        "one_at_a_time_full_runs": full,
     }
 
 
@@ -310,13 +428,15 @@ def copy_calibration(mib=256, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="solvated_chain_100000")
     ap.add_argument("--skin", type=float, default=0.1)
     ap.add_argument("--relax", type=int, default=300, help="steepest-descent iterations before timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving measurement")
     ap.add_argument("--serving-only", action="store_true", help="run only the batched-serving measurement")
     ap.add_argument("--serving-requests", type=int, default=256)
@@ -368,27 +488,35 @@ def main():
         c.close()
     log("force errors", errs)
 
-    ctx_m, sec_m, kern_m, stats_m, pairs, graph_m, timed_m = time_path(sysm, L.MIXED, args.steps, args.warmup,
-                                                                       args.skin)
+    ctx_m, sec_m, kern_m, stats_m, pairs, pair_m = time_path(sysm, L.MIXED, args.steps, args.warmup, args.skin)
     ctx_m.close()
     nsd_m = ns_per_day(args.steps, sysm.dt, sec_m)
     log("mixed: %.3f ms/step, %.2f ns/day" % (1e3 * sec_m / args.steps, nsd_m))
     fp64 = None
     if not args.no_fp64:
-        steps64 = max(10, args.steps // 4)
-        ctx_d, sec_d, kern_d, stats_d, _, graph_d, timed_d = time_path(sysm, L.FP64, steps64,
-                                                                       max(5, args.warmup // 4), args.skin)
+        ctx_d, sec_d, kern_d, stats_d, _, pair_d = time_path(sysm, L.FP64, args.steps, args.warmup, args.skin)
         ctx_d.close()
-        nsd_d = ns_per_day(steps64, sysm.dt, sec_d)
-        fp64 = {"ns_per_day": nsd_d, "ms_per_step": 1e3 * sec_d / steps64, "steps": steps64,
-                "graph_replay_ns_per_day": ns_per_day(steps64, sysm.dt, graph_d),
-                "pair_ms_per_launch": timed_d["pair_ms_per_launch"], "kernels": kern_d}
-        log("fp64: %.3f ms/step, %.2f ns/day" % (1e3 * sec_d / steps64, nsd_d))
+        nsd_d = ns_per_day(args.steps, sysm.dt, sec_d)
+        fp64 = {"ns_per_day": nsd_d, "ms_per_step": 1e3 * sec_d / args.steps, "steps": args.steps,
+                "pair": pair_d, "kernels": kern_d}
+        log("fp64: %.3f ms/step, %.2f ns/day" % (1e3 * sec_d / args.steps, nsd_d))
+    steady = {"mixed": steady_state(sysm, L.MIXED, args.skin)}
+    if fp64:
+        steady["fp64"] = steady_state(sysm, L.FP64, args.skin)
+        steady["mixed_over_fp64"] = steady["mixed"]["ns_per_day"] / steady["fp64"]["ns_per_day"]
+    log("steady state", steady)
+    drift = None
+    if not args.no_drift:
+        drift = {"mixed": energy_drift(sysm, L.MIXED, args.skin)}
+        if fp64:
+            drift["fp64"] = energy_drift(sysm, L.FP64, args.skin)
+        log("drift", drift)
 
-    # pair-kernel roofline (FP32 VALU; mixed path)
-    pair_ms = timed_m["pair_ms_per_launch"]
+    # pair-kernel roofline (FP32 VALU; mixed path), from the launches beside PME
+    pair_ms = pair_m["beside_pme_ms_per_launch"]
     flops = PAIR_FLOP * pairs
     achieved = flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else 0.0
+    alone_ms = pair_m["alone_ms_per_launch"]
     tiles = stats_m["tiles"]
     roofline = {
         "kernel": "pair (real-space LJ + erfc Coulomb, mixed)",
@@ -400,6 +528,11 @@ def main():
         "traffic": profiled_traffic(),
         "traffic_source": "profiles/kernels_solvated_chain_100000.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                           "scaled by a 256 MiB copy)",
+        "frac_uses": "the pair kernel as it runs in a step, beside PME: %d launches with events on its stream"
+                     % pair_m["beside_pme_launches"],
+        "ms_per_launch": pair_ms,
+        "ms_per_launch_alone": alone_ms,
+        "achieved_alone": flops / (alone_ms * 1e-3) / 1e12 if alone_ms > 0 else 0.0,
         "peak_without_packed_fp32": PEAK_FP32_VALU / 2,
         "valu_busy": profiled_valu_busy(),
         "valu_busy_source": "profiles/valu_pair_mixed_solvated_chain_100000.md (rocprofv3 --pmc "
@@ -407,7 +540,6 @@ def main():
         "algorithmic_flop_per_launch": flops,
         "pairs_within_cutoff": pairs,
         "pairs_evaluated": tiles * 64,
-        "ms_per_launch": pair_ms,
     }
     ex = O.exclusions(sysm)
     n_excl = int(len(ex[1]) // 2)
@@ -434,23 +566,21 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (forces in 64-bit fixed point, fp64 state and energy sums)",
         "data": "synthetic (seeded generator; random-walk chain + jittered-lattice water)",
-        "config": {"workload": args.workload, "atoms": sysm.n, "box_nm": float(sysm.L[0]),
-                   "pme_grid": list(sysm.grid), "pme_order": 5, "cutoff_nm": sysm.cutoff,
-                   "ewald_tol": sysm.ewald_tol, "dt_fs": sysm.dt * 1e3, "skin_nm": args.skin,
-                   "precision": "mixed"},
+        "config": workload_config(args.workload, sysm, args.skin),
         "fp64": fp64,
-        "graph_replay_ns_per_day": ns_per_day(args.steps, sysm.dt, graph_m),
         "mixed_over_fp64": (nsd_m / fp64["ns_per_day"]) if fp64 else None,
+        "steady_state": steady,
+        "energy_drift": drift,
         "force_error_vs_fp64_oracle": errs,
         "roofline": roofline,
+        "pair_timing": pair_m,
         "kernels_mixed": kernels,
-        "timed_region": timed_m,
-        "pair_list": {"entries": stats_m["list_entries"], "tiles": tiles, "rebuilds": stats_m["rebuilds"]},
+        "pair_list": {"entries": stats_m["list_entries"], "tiles": tiles, "rebuilds_in_timed_region": stats_m["rebuilds"]},
     }
     if not args.no_cpu:
-        cb = cpu_baseline(sysm)
+        cb = cpu_baseline(sysm, args.cpu_steps)
         out["cpu_baseline"] = cb
-        log("cpu baseline %.4f ns/day" % cb["value"])
+        log("cpu baseline %.4f ns/day (%s)" % (cb["value"], cb["kind"]))
     if not args.no_serving:
         out["serving"] = serving_bench(args.serving_requests, args.serving_steps, args.alone_steps)
         log("serving", out["serving"])

@@ -139,13 +139,11 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
  *                         the other on one stream, with events around it:
  *                         "nlist", "pack", "bonded", "atom_forces", "pair",
  *                         "pme_spread", "fft", "pme_gather", "integrate"
- *   NX_TIMING_GRAPH_PAIR  nx_step replays every step as one graph, except
- *                         every 128th (the first included), which replays as
- *                         graphs around the pair kernel (list before it, terms
- *                         and PME beside it, update after it) with events on
- *                         the main stream around the pair kernel; an event
- *                         closes every block of 128 steps and the last step:
- *                         "pair" (sampled steps) and "step" (per step)
+ *   NX_TIMING_GRAPH_PAIR  nx_step replays every step as graphs around the pair
+ *                         kernel (list before it, terms and PME beside it,
+ *                         update after it) with events on the main stream
+ *                         around the pair kernel and after every step: "pair"
+ *                         (the pair kernel as it runs beside PME) and "step"
  * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
  * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);
  * NEXUSMD_OVERLAP=0 keeps every kernel on one stream.
@@ -179,7 +177,10 @@ nx_status nx_fft3d_c2r(int device, int precision, int nx, int ny, int nz, const 
 typedef struct nx_server nx_server;
 typedef struct {
     int64_t submitted, admitted, finished, failed, active;
-    double p50_step_ms, p99_step_ms;      /* per-request step latency */
+    double p50_step_ms, p99_step_ms;      /* per (request, step) latency: device time between the ends of
+                                             successive batched steps (events on the batch stream); the
+                                             first step after retirement, admission or a re-layout also
+                                             counts the host time those took */
     double aggregate_ns_per_day;          /* sum of simulated time / wall time */
     double wall_s;
     int64_t batched_steps;                /* batched device steps executed */

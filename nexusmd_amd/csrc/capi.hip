@@ -77,6 +77,10 @@ void check_topology(const nx_topology* t) {
     if (!(o.ewald_tolerance > 0 && o.ewald_tolerance < 0.5)) throw ArgError("ewald_tolerance must be in (0, 0.5)");
     if (o.pme_order != 0 && o.pme_order != 5) throw ArgError("pme_order must be 5");
     if (o.skin < 0 || o.skin > 1.0) throw ArgError("skin must be in [0, 1] nm");
+    // the device applies the switch from r_switch >= 0 (a switch distance at or
+    // beyond the cutoff leaves every pair unswitched, as in the reference)
+    if (o.use_switch && !(std::isfinite(o.switch_distance) && o.switch_distance >= 0))
+        throw ArgError("use_switch needs a finite switch_distance >= 0");
     if (t->excl_offsets && t->excl_indices) {
         const int* off = t->excl_offsets;
         if (off[0] != 0) throw ArgError("excl_offsets[0] must be 0");
@@ -117,6 +121,28 @@ void copy_topology(nx::HostSystem& h, const nx_topology* t) {
 void check_positions(int n, const double* p) {
     for (int i = 0; i < 3 * n; ++i)
         if (!std::isfinite(p[i])) throw ArgError("non-finite position or velocity component " + std::to_string(i));
+}
+
+// After a list overflow inside a call the device froze the system at the last
+// step it completed.  Its state becomes the host copy and the next call lays
+// the context out again with twice the list room, from that state.
+template <class F>
+void with_capacity_recovery(nx_ctx* c, F&& f) {
+    try {
+        f();
+    } catch (const nx::HipError& e) {
+        if (e.code == NX_ERR_CAPACITY && c->eng && !c->dirty) {
+            try {
+                c->eng->device_state(0, c->sys.pos.data(), c->sys.vel.data());
+                c->sys.build_pos.clear();
+                c->sys.rebuild_pending = false;
+                c->sys.list_scale *= 2.0;
+                c->dirty = true;
+            } catch (...) {
+            }
+        }
+        throw;
+    }
 }
 
 void ensure(nx_ctx* c) {
@@ -235,6 +261,13 @@ nx_status nx_set_state(nx_ctx* c, const double* pos, const double* vel) {
 
 nx_status nx_get_state(nx_ctx* c, double* pos, double* vel, double* forces) {
     return guarded(c, [&] {
+        if (c->dirty && c->have_top && c->have_state && !forces) {
+            // not laid out on the device (yet, or again after a capacity error):
+            // the host copy is the current state
+            if (pos) std::memcpy(pos, c->sys.pos.data(), c->sys.pos.size() * sizeof(double));
+            if (vel) std::memcpy(vel, c->sys.vel.data(), c->sys.vel.size() * sizeof(double));
+            return;
+        }
         ensure(c);
         std::vector<double> p, v;
         c->eng->get_state(pos ? &p : nullptr, vel ? &v : nullptr);
@@ -254,7 +287,7 @@ nx_status nx_compute_forces(nx_ctx* c, int mask, double* forces, nx_energies* e)
         ensure(c);
         std::vector<double> F;
         std::vector<nx_energies> E;
-        c->eng->compute(mask, true, 0.0, false, forces ? &F : nullptr, nullptr, nullptr, &E);
+        with_capacity_recovery(c, [&] { c->eng->compute(mask, true, 0.0, false, forces ? &F : nullptr, nullptr, nullptr, &E); });
         if (forces) std::memcpy(forces, F.data(), F.size() * sizeof(double));
         if (e) *e = E[0];
         if (e && !std::isfinite(e->potential)) throw std::runtime_error("non-finite energy");
@@ -266,7 +299,7 @@ nx_status nx_step(nx_ctx* c, int nsteps, double dt, int mask) {
         if (nsteps < 0 || !(dt > 0) || !std::isfinite(dt)) throw ArgError("nsteps >= 0 and dt > 0 required");
         if (mask & ~NX_FORCE_ALL) throw ArgError("unknown force bits");
         ensure(c);
-        if (nsteps) c->eng->run(nsteps, dt, mask);
+        if (nsteps) with_capacity_recovery(c, [&] { c->eng->run(nsteps, dt, mask); });
     });
 }
 
@@ -277,7 +310,9 @@ nx_status nx_step_report(nx_ctx* c, double dt, int mask, double* pos, double* ve
         ensure(c);
         std::vector<double> F, p, v;
         std::vector<nx_energies> E;
-        c->eng->compute(mask, true, dt, true, forces ? &F : nullptr, pos ? &p : nullptr, vel ? &v : nullptr, &E);
+        with_capacity_recovery(c, [&] {
+            c->eng->compute(mask, true, dt, true, forces ? &F : nullptr, pos ? &p : nullptr, vel ? &v : nullptr, &E);
+        });
         if (forces) std::memcpy(forces, F.data(), F.size() * sizeof(double));
         if (pos) std::memcpy(pos, p.data(), p.size() * sizeof(double));
         if (vel) std::memcpy(vel, v.data(), v.size() * sizeof(double));

@@ -166,6 +166,7 @@ class Engine : public EngineBase {
         if (fork_) hipEventDestroy(fork_);
         if (join_) hipEventDestroy(join_);
         for (auto& e : evpool_) hipEventDestroy(e);
+        for (auto& e : step_ev_) hipEventDestroy(e);
     }
 
     void release() {
@@ -622,17 +623,31 @@ class Engine : public EngineBase {
         std::vector<SysCount> c(ns);
         copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * ns, hipMemcpyDeviceToHost);
         bool grow = false;
+        failed_sys_.clear();
+        for (int s = 0; s < ns; ++s)
+            if (c[s].overflow & 2) failed_sys_.push_back(s);
+        if (!failed_sys_.empty()) {
+            grow_depth_ = 0;
+            throw HipError(NX_ERR_CAPACITY, "neighbour-list column exceeds 2048 atoms in system " +
+                                                std::to_string(failed_sys_[0]));
+        }
+        std::vector<int> growing;
         for (int s = 0; s < ns; ++s) {
-            if (c[s].overflow & 2) throw HipError(NX_ERR_CAPACITY, "neighbour-list column exceeds 2048 atoms");
-            if (c[s].overflow) { hs_[s]->list_scale *= 2.0; grow = true; }
+            if (c[s].overflow) { hs_[s]->list_scale *= 2.0; grow = true; growing.push_back(s); }
             else if (c[s].list_count > 0.8 * B.sys[s].list_cap || c[s].emask_count > 0.8 * B.sys[s].emask_cap) {
                 hs_[s]->list_scale *= 1.6;
                 grow = true;
+                growing.push_back(s);
             }
         }
         counts_ = c;
         if (grow) {
-            if (++grow_depth_ > 6) throw HipError(NX_ERR_CAPACITY, "pair list does not fit");
+            if (++grow_depth_ > 6) {
+                // the systems that still do not fit are named as failed
+                grow_depth_ = 0;
+                failed_sys_ = growing;
+                throw HipError(NX_ERR_CAPACITY, "pair list does not fit in system " + std::to_string(growing[0]));
+            }
             std::vector<HostSystem*> keep = hs_;
             setup(keep);
         }
@@ -793,16 +808,21 @@ class Engine : public EngineBase {
         check_counters();
     }
 
+    // Every system whose list build overflowed during the call is named (the
+    // device froze each at its last completed step); the error names the first.
     void check_counters() {
         std::vector<SysCount> c(B.nsys);
         copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * B.nsys, hipMemcpyDeviceToHost);
-        for (int s = 0; s < B.nsys; ++s)
-            if (c[s].overflow) {
-                failed_sys_ = s;
-                throw HipError(NX_ERR_CAPACITY, "neighbour-list capacity exceeded in system " + std::to_string(s) +
-                                                    " (code " + std::to_string(c[s].overflow) + ")");
-            }
         counts_ = c;
+        failed_sys_.clear();
+        for (int s = 0; s < B.nsys; ++s)
+            if (c[s].overflow) failed_sys_.push_back(s);
+        if (!failed_sys_.empty()) {
+            const int s = failed_sys_[0];
+            throw HipError(NX_ERR_CAPACITY, "neighbour-list capacity exceeded in system " + std::to_string(s) +
+                                                " (code " + std::to_string(c[s].overflow) + ", " +
+                                                std::to_string(failed_sys_.size()) + " system(s) failed)");
+        }
     }
 
     void step_plain(double dt, int mask) {
@@ -815,46 +835,32 @@ class Engine : public EngineBase {
         begin_call();
         settle();
         if (B.timing == NX_TIMING_GRAPH_PAIR && nsteps > 0 && graphs_enabled_) {
-            // Every kPairSampleEvery-th step (the first included) replays as graphs
-            // around the pair kernel -- [list, positions], the side graph (terms,
-            // PME; on the side stream beside the pair kernel), the pair kernel,
-            // [update] -- with events on the main stream around the pair kernel;
-            // the other steps replay the whole step as one graph.  An event on the
-            // main stream closes every block of kPairSampleEvery steps (and the
-            // last step); a marker after every step would add its own latency to
-            // every step.
+            // Every step replays as graphs around the pair kernel -- [list,
+            // positions], the side graph (terms, PME; on the side stream beside the
+            // pair kernel), the pair kernel, [update] -- with events on the main
+            // stream around the pair kernel and one after every step.
             if (!seg_exec_[0] || graph_dt_seg_ != dt || graph_mask_seg_ != mask) build_segments(dt, mask);
-            if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
             const bool side = use_side(mask);
             hipEvent_t e0 = ev();
             HIPCHK(hipEventRecord(e0, B.stream));
-            int n0 = 0;   // first step of the open block
             for (int i = 0; i < nsteps; ++i) {
-                if (i % kPairSampleEvery == 0) {
-                    hipEvent_t a = ev(), b = ev();
-                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
-                    if (side) {
-                        fork_side();
-                        HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], side_));
-                    } else {
-                        HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
-                    }
-                    HIPCHK(hipEventRecord(a, B.stream));
-                    if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
-                    HIPCHK(hipEventRecord(b, B.stream));
-                    if (side) join_side();
-                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_LAST], B.stream));
-                    evs_.push_back({"pair", a, b});
+                hipEvent_t a = ev(), b = ev(), c = ev();
+                HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
+                if (side) {
+                    fork_side();
+                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], side_));
                 } else {
-                    HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
+                    HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
                 }
-                if ((i + 1) % kPairSampleEvery == 0 || i + 1 == nsteps) {
-                    hipEvent_t c = ev();
-                    HIPCHK(hipEventRecord(c, B.stream));
-                    evs_.push_back({"step", e0, c, i + 1 - n0});
-                    e0 = c;
-                    n0 = i + 1;
-                }
+                HIPCHK(hipEventRecord(a, B.stream));
+                if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
+                HIPCHK(hipEventRecord(b, B.stream));
+                if (side) join_side();
+                HIPCHK(hipGraphLaunch(seg_exec_[SEG_LAST], B.stream));
+                HIPCHK(hipEventRecord(c, B.stream));
+                evs_.push_back({"pair", a, b});
+                evs_.push_back({"step", e0, c});
+                e0 = c;
             }
             steps_ += nsteps;
         } else if (B.timing) {
@@ -869,6 +875,43 @@ class Engine : public EngineBase {
             for (int i = 0; i < nsteps; ++i) step_plain(dt, mask);
         }
         end_call();
+    }
+
+    // nsteps steps (graph replays where run() would use them) with an event on
+    // the main stream before the first step and after every step; ms[i] is the
+    // device time from the end of step i - 1 (the first: from the start) to the
+    // end of step i.
+    void run_step_times(int nsteps, double dt, int mask, std::vector<double>& ms) override {
+        ms.clear();
+        if (nsteps <= 0) return;
+        begin_call();
+        settle();
+        const bool graph = nsteps >= 4 && graphs_enabled_;
+        if (graph && (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask)) build_graph(dt, mask);
+        if ((int)step_ev_.size() < nsteps + 1) {
+            for (int i = (int)step_ev_.size(); i < nsteps + 1; ++i) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                step_ev_.push_back(e);
+            }
+        }
+        HIPCHK(hipEventRecord(step_ev_[0], B.stream));
+        for (int i = 0; i < nsteps; ++i) {
+            if (graph) {
+                HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
+                ++steps_;
+            } else {
+                step_plain(dt, mask);
+            }
+            HIPCHK(hipEventRecord(step_ev_[i + 1], B.stream));
+        }
+        end_call();
+        ms.resize(nsteps);
+        for (int i = 0; i < nsteps; ++i) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, step_ev_[i], step_ev_[i + 1]));
+            ms[i] = t;
+        }
     }
 
     void build_graph(double dt, int mask) {
@@ -1082,7 +1125,7 @@ class Engine : public EngineBase {
     void set_timing(int mode) override { B.timing = mode; }
     void set_graphs(bool on) override { graphs_enabled_ = on; }
     int nsys() const override { return B.nsys; }
-    int failed_system() const override { return failed_sys_; }
+    const std::vector<int>& failed_systems() const override { return failed_sys_; }
     const std::vector<SysCount>& counters() const override { return counts_; }
     hipStream_t stream() const override { return B.stream; }
     void device_state(int s, double* pos, double* vel) override {
@@ -1128,6 +1171,7 @@ class Engine : public EngineBase {
     int fft_maxK_ = 0;
     long long* d_count_ = nullptr;
     std::vector<hipEvent_t> evpool_;
+    std::vector<hipEvent_t> step_ev_;       // run_step_times: one event per step boundary
     size_t ev_used_ = 0;
     std::vector<Ev> evs_;
     std::map<std::string, std::pair<double, long long>> times_;
@@ -1138,11 +1182,10 @@ class Engine : public EngineBase {
     hipGraphExec_t graph_exec_ = nullptr;
     double graph_dt_ = 0.0;
     int graph_mask_ = -1;
-    int failed_sys_ = -1;
+    std::vector<int> failed_sys_;           // systems whose list overflowed in the last call
     int grow_depth_ = 0;
     // timing segments: [list, positions], [terms, PME], [pair], [update]
     enum { SEG_FIRST, SEG_SIDE, SEG_PAIR, SEG_LAST, SEG_COUNT };
-    static constexpr int kPairSampleEvery = 128;
     hipGraph_t seg_[SEG_COUNT] = {};
     hipGraphExec_t seg_exec_[SEG_COUNT] = {};
     hipStream_t side_ = nullptr;            // PME beside the pair kernel

@@ -121,9 +121,13 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
         FSx[sl] = 0;
         FSy[sl] = 0;
         FSz[sl] = 0;
+        // a system whose neighbour-list build overflowed lacks pair forces: it is
+        // frozen at the last step it completed (the host reports it as failed)
+        const bool frozen = sc[s].overflow != 0;
+        if (a == d.atom_off && !frozen) sc[s].steps_done += 1;
         const double im = invm[a];
         double x = X[a], y = Y[a], z = Z[a];
-        if (im != 0.0) {
+        if (im != 0.0 && !frozen) {
             double vx = VX[a] + im * f[0] * dt;
             double vy = VY[a] + im * f[1] * dt;
             double vz = VZ[a] + im * f[2] * dt;

@@ -650,7 +650,9 @@ __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc,
             for (int s = 0; s < nsys; ++s) {
                 prefix[s] = t;
                 if (s < kWavesLdsSys) { pre_s[s] = t; off_s[s] = sd[s].chunk_off; }
-                t += min(sc[s].n_chunks, sd[s].chunk_cap);     // the counter runs past the capacity on overflow
+                // a system whose build overflowed has an incomplete list (a block that
+                // overflowed reserved chunk slots it did not write): no pair work
+                if (!sc[s].overflow) t += min(sc[s].n_chunks, sd[s].chunk_cap);
             }
             prefix[nsys] = t;
             if (nsys <= kWavesLdsSys) pre_s[nsys] = t;

@@ -228,11 +228,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
     if (P::mixed && g < hi) cur = fetch(g);
     while (g < hi) {
         if constexpr (!P::mixed) cur = fetch(g);
-        int tk = 0;     // the claim of the chunk after this one
         // chunk values are the same on every lane: keep them in scalar registers
         const int s = __builtin_amdgcn_readfirstlane(cur.s);
         const int fc = __builtin_amdgcn_readfirstlane(cur.ch.x), ncl = __builtin_amdgcn_readfirstlane(cur.ch.y);
         const int e0 = __builtin_amdgcn_readfirstlane(cur.ch.z), e1 = __builtin_amdgcn_readfirstlane(cur.ch.w);
+        // the claim of the chunk after this one: inside the entry loop, or here for
+        // a chunk without entries (the loop below then does not run)
+        int tk = 0;
+        if (e1 <= e0) tk = claim();
         const real Lx = (real)sd[s].L[0], Ly = (real)sd[s].L[1], Lz = (real)sd[s].L[2];
         const real iLx = (real)sd[s].invL[0], iLy = (real)sd[s].invL[1], iLz = (real)sd[s].invL[2];
         const real rc2 = (real)sd[s].rc2;

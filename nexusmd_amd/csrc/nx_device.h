@@ -63,7 +63,7 @@ struct SysDev {
 struct SysCount {
     int n_slots, n_clusters, n_super, list_count;
     int emask_count, rebuild, overflow, rebuilds;
-    int n_chunks, pad0, pad1, pad2;
+    int n_chunks, steps_done, pad1, pad2;   // steps_done: leapfrog steps the system completed
 };
 
 constexpr int kChunkEntries = 32;       // pair-list entries per chunk, the pair kernel's unit of work

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/nexusmd.h"
+
 #include "nx_device.h"
 
 namespace nx {
@@ -150,5 +152,4 @@ template <class P> void launch_export_pairs(Batch<P>& B, int2* out, long long ca
 // standalone FFT (verification)
 template <class P> void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool forward,
                                         hipStream_t st);
-
 }  // namespace nx

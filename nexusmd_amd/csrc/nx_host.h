@@ -47,6 +47,8 @@ class EngineBase {
     virtual ~EngineBase() = default;
     virtual void setup(const std::vector<HostSystem*>& systems) = 0;
     virtual void run(int nsteps, double dt, int mask) = 0;
+    // as run, with the device time of every step (ms, from the end of the previous step)
+    virtual void run_step_times(int nsteps, double dt, int mask, std::vector<double>& ms) = 0;
     virtual void compute(int mask, bool kinetic, double dt, bool integrate, std::vector<double>* F,
                          std::vector<double>* pos, std::vector<double>* vel, std::vector<nx_energies>* E) = 0;
     virtual void get_state(std::vector<double>* pos, std::vector<double>* vel) = 0;
@@ -57,7 +59,7 @@ class EngineBase {
     virtual void set_timing(int mode) = 0;   // NX_TIMING_*
     virtual void set_graphs(bool on) = 0;
     virtual int nsys() const = 0;
-    virtual int failed_system() const = 0;
+    virtual const std::vector<int>& failed_systems() const = 0;   // overflowed in the last call
     virtual const std::vector<SysCount>& counters() const = 0;
     virtual hipStream_t stream() const = 0;
     virtual void device_state(int s, double* pos, double* vel) = 0;

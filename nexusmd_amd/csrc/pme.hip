@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -95,7 +97,7 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
     if (slot >= n_slots_total) return;
     const int s = slot_sys[slot];
     const SysDev& d = sd[s];
-    if (slot - d.slot_off >= sc[s].n_slots) return;
+    if (slot - d.slot_off >= sc[s].n_slots || sc[s].overflow) return;
     if (atom_of_slot[slot] < 0) return;
     if (d.grid[0] == 0 || d.sp_block[0] > 0) return;
     const typename P::real4 p = pos[slot];
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
     const int4 t = tiles[blockIdx.x];          // (sys, bx, by, bz)
     const int s = t.x;
     const SysDev& d = sd[s];
-    if (sc[s].n_slots == 0) return;
+    if (sc[s].n_slots == 0 || sc[s].overflow) return;     // overflowed build: the system is frozen
     const int Kx = d.grid[0], Ky = d.grid[1], Kz = d.grid[2];
     const int BX = d.sp_block[0], BY = d.sp_block[1], NZ = d.sp_block[2];
     const int cx0 = t.y * BX, cx1 = min(d.ncx, cx0 + BX);
@@ -316,7 +318,8 @@ __global__ __launch_bounds__(5 * kGatherSlots) void pme_gather_kernel(const SysD
     int s = 0;
     if (on) {
         s = slot_sys[slot];
-        on = slot - sd[s].slot_off < sc[s].n_slots && atom_of_slot[slot] >= 0 && sd[s].grid[0] != 0;
+        on = slot - sd[s].slot_off < sc[s].n_slots && !sc[s].overflow && atom_of_slot[slot] >= 0 &&
+             sd[s].grid[0] != 0;
     }
     real fx = 0, fy = 0, fz = 0;
     typename P::real4 p;

@@ -130,9 +130,33 @@ void relayout(nx_server* s) {
     if (!s->dirty) return;
     if (s->active.empty()) { s->dirty = false; return; }
     if (!s->eng) s->eng.reset(nx::make_engine(s->device, s->precision));
-    std::vector<nx::HostSystem*> v;
-    for (int64_t id : s->active) v.push_back(&s->reqs[id]->sys);
-    s->eng->setup(v);
+    // A request whose first list build cannot fit fails alone: it leaves the
+    // batch and the others are laid out again.
+    while (!s->active.empty()) {
+        std::vector<nx::HostSystem*> v;
+        for (int64_t id : s->active) v.push_back(&s->reqs[id]->sys);
+        try {
+            s->eng->setup(v);
+            break;
+        } catch (const nx::HipError& e) {
+            const std::vector<int> bad = s->eng->failed_systems();
+            if (e.code != NX_ERR_CAPACITY || bad.empty()) throw;
+            std::vector<int64_t> keep;
+            for (size_t i = 0; i < s->active.size(); ++i) {
+                Request& r = *s->reqs[s->active[i]];
+                if (std::find(bad.begin(), bad.end(), (int)i) == bad.end()) {
+                    keep.push_back(r.id);
+                    continue;
+                }
+                r.status = 3;
+                r.error = std::string("admission failed: ") + e.what();
+                s->failed++;
+                s->in_use -= (int64_t)r.bytes;
+            }
+            if (keep.size() == s->active.size()) throw;
+            s->active = keep;
+        }
+    }
     s->dirty = false;
 }
 
@@ -244,11 +268,17 @@ nx_status nx_submit(nx_server* s, const nx_topology* t, const double lb[3], cons
 nx_status nx_server_run(nx_server* s, int steps) {
     return sguard(s, [&] {
         int budget_steps = steps > 0 ? steps : 0x7fffffff;
+        using clock = std::chrono::steady_clock;
+        // host time of the last step completion seen (or of this call's start):
+        // retirement, admission and re-layout before a chunk count toward the
+        // latency of the chunk's first step
+        auto t_prev = clock::now();
         while (budget_steps > 0) {
             if (s->eng && !s->dirty) retire_finished(s);
             admit(s);
             if (s->active.empty()) break;
             relayout(s);
+            if (s->active.empty()) continue;     // every admitted request failed its layout
             // chunk: no active request may overshoot; one dt per chunk group
             int k = std::min(budget_steps, s->chunk);
             double dt = s->reqs[s->active[0]]->dt;
@@ -258,33 +288,59 @@ nx_status nx_server_run(nx_server* s, int steps) {
                 if (r.dt != dt) throw std::invalid_argument("all active requests must share one dt");
             }
             if (k <= 0) { retire_finished(s); continue; }
-            const auto t0 = std::chrono::steady_clock::now();
+            // steps each system had completed before this chunk (device counters)
+            std::vector<int> before(s->active.size(), 0);
+            {
+                const auto& c0 = s->eng->counters();
+                for (size_t i = 0; i < before.size() && i < c0.size(); ++i) before[i] = c0[i].steps_done;
+            }
+            const auto t0 = clock::now();
+            std::vector<double> step_ms;
             try {
-                s->eng->run(k, dt, NX_FORCE_ALL);
+                s->eng->run_step_times(k, dt, NX_FORCE_ALL, step_ms);
             } catch (const nx::HipError& e) {
-                const int f = s->eng->failed_system();
-                if (e.code != NX_ERR_CAPACITY || f < 0 || f >= (int)s->active.size()) throw;
-                // isolate the failing request; the others completed the k steps
+                const std::vector<int> bad_idx = s->eng->failed_systems();
+                if (e.code != NX_ERR_CAPACITY || bad_idx.empty()) throw;
+                for (int f : bad_idx)
+                    if (f < 0 || f >= (int)s->active.size()) throw;
+                // Isolate every request whose list overflowed: the device froze each
+                // at its last completed step.  The others completed the k steps.
                 sync_active_to_host(s);
-                for (size_t i = 0; i < s->active.size(); ++i)
-                    if ((int)i != f) s->reqs[s->active[i]]->done += k;
-                Request& bad = *s->reqs[s->active[f]];
-                bad.status = 3;
-                bad.error = e.what();
-                s->failed++;
-                s->in_use -= (int64_t)bad.bytes;
-                s->active.erase(s->active.begin() + f);
+                const auto& c1 = s->eng->counters();
+                std::vector<int64_t> keep;
+                for (size_t i = 0; i < s->active.size(); ++i) {
+                    Request& r = *s->reqs[s->active[i]];
+                    if (std::find(bad_idx.begin(), bad_idx.end(), (int)i) == bad_idx.end()) {
+                        r.done += k;
+                        keep.push_back(r.id);
+                        continue;
+                    }
+                    r.done += std::max(0, c1[i].steps_done - before[i]);
+                    r.status = 3;
+                    r.error = std::string(e.what()) + " (this request: system " + std::to_string(i) + ", code " +
+                              std::to_string(c1[i].overflow) + ")";
+                    s->failed++;
+                    s->in_use -= (int64_t)r.bytes;
+                }
+                s->active = keep;
                 s->dirty = true;
+                t_prev = clock::now();
                 continue;
             }
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            s->wall_s += ms * 1e-3;
+            const auto t1 = clock::now();
+            const double stall_ms = std::chrono::duration<double, std::milli>(t0 - t_prev).count();
+            s->wall_s += std::chrono::duration<double>(t1 - t_prev).count();
+            t_prev = t1;
             s->batched_steps += k;
+            // one latency sample per (request, step): the device time between the
+            // ends of successive batched steps, the first step of the chunk with
+            // the host stall before it
+            if (!step_ms.empty()) step_ms[0] += stall_ms;
             for (int64_t id : s->active) {
                 Request& r = *s->reqs[id];
                 r.done += k;
                 s->sim_ns += k * r.dt * 1e-3;
-                for (int i = 0; i < k; ++i) s->step_ms.push_back(ms / k);
+                s->step_ms.insert(s->step_ms.end(), step_ms.begin(), step_ms.end());
             }
             budget_steps -= k;
         }

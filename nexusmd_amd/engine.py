@@ -69,6 +69,7 @@ class Context:
 
     def __init__(self, sys, precision=L.FP64, device=0, skin=0.1, excl=None, use_switch=0, switch_distance=0.0,
                  grid=None, box=None):
+        self.h = None
         self.lib = L.lib()
         self.sys = sys
         self.n = sys.n
@@ -204,8 +205,8 @@ class Server:
         if st != L.NX_OK:
             raise L.NexusError(st, "%s failed: %s" % (what, self.lib.nx_server_last_error(self.h).decode()))
 
-    def submit(self, sys, nsteps, dt, skin=0.1):
-        tb = TopologyBuffers(sys, skin=skin)
+    def submit(self, sys, nsteps, dt, skin=0.1, excl=None):
+        tb = TopologyBuffers(sys, skin=skin, excl=excl)
         lb = np.ascontiguousarray(sys.lb, np.float64)
         ub = np.ascontiguousarray(sys.ub, np.float64)
         p = np.ascontiguousarray(sys.pos, np.float64)

@@ -133,3 +133,56 @@ def test_bad_request_is_rejected_without_disturbing_others(requests):
     srv.run()
     assert srv.poll(good) == (2, k0)
     srv.close()
+
+
+def test_mid_run_overflow_fails_only_that_request(requests):
+    """A request whose neighbour-list build overflows mid-run (every atom of a
+    collapsing cloud lands in one column after about 34 steps) fails alone:
+    the device freezes it at its last completed step, the server names it
+    failed, and the requests beside it finish exactly as they do alone
+    (mixed path, bit for bit)."""
+    from _util import collapse_system
+    bad, excl = collapse_system()
+    others = [requests[0], requests[2], requests[4]]
+    print("[gpu] serving mid-run overflow: collapse n=%d beside %s" % (bad.n, [s.n for s, _ in others]), flush=True)
+    srv = E.Server(precision=L.MIXED, budget_bytes=8 << 30, max_active=8)
+    ids = [srv.submit(s, k, DT) for s, k in others]
+    rid = srv.submit(bad, 80, DT, excl=excl)
+    srv.run()
+    st, done = srv.poll(rid)
+    assert st == 3, "the collapsing request must fail"
+    assert 20 <= done < 80
+    with pytest.raises(L.NexusError) as ex:
+        srv.fetch(rid)
+    assert ex.value.status == L.NX_ERR_CAPACITY
+    out = []
+    for i, (s, k) in zip(ids, others):
+        assert srv.poll(i) == (2, k)
+        out.append(srv.fetch(i))
+    stats = srv.stats()
+    assert stats["failed"] == 1 and stats["finished"] == len(others)
+    srv.close()
+    for (s, k), (p, v, e) in zip(others, out):
+        pa, va, ea = _alone(s, k, L.MIXED)
+        np.testing.assert_array_equal(p, pa)
+        np.testing.assert_array_equal(v, va)
+        assert e["potential"] == ea["potential"]
+
+
+def test_two_overflowing_requests_both_fail(requests):
+    """Two requests that overflow in the same chunk are both named failed; the
+    third request finishes."""
+    from _util import collapse_system
+    a, ea = collapse_system(seed=5)
+    b, eb = collapse_system(seed=6)
+    s0, k0 = requests[1]
+    print("[gpu] serving two overflows beside n=%d" % s0.n, flush=True)
+    srv = E.Server(precision=L.MIXED, budget_bytes=8 << 30, max_active=8)
+    good = srv.submit(s0, k0, DT)
+    ra = srv.submit(a, 60, DT, excl=ea)
+    rb = srv.submit(b, 60, DT, excl=eb)
+    srv.run()
+    assert srv.poll(ra)[0] == 3 and srv.poll(rb)[0] == 3
+    assert srv.poll(good) == (2, k0)
+    assert srv.stats()["failed"] == 2
+    srv.close()
