@@ -154,7 +154,8 @@ int64_t nxo_pairs_voxel(int n, const double* pos, const nxo_box* box, const int*
     const int W0 = nv[0] + 2, W1 = nv[1] + 2, W2 = nv[2] + 2;   /* keys -1..n */
     int64_t nvox = (int64_t)W0 * W1 * W2;
     int* head = (int*)malloc(sizeof(int) * nvox);
-    for (int64_t v = 0; v < nvox; ++v) head[v] = -1;
+    int* tail = (int*)malloc(sizeof(int) * nvox);
+    for (int64_t v = 0; v < nvox; ++v) head[v] = tail[v] = -1;
     int* next = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
     int* key = (int*)malloc(sizeof(int) * 3 * (n > 0 ? n : 1));
     const double max2 = cutoff * cutoff;
@@ -194,17 +195,20 @@ int64_t nxo_pairs_voxel(int n, const double* pos, const nxo_box* box, const int*
                 }
             }
         }
-        /* insert after the query (:151-152) */
+        /* insert after the query (:151-152), at the end of the voxel's list: a
+           Voxel is a vector filled by push_back (:37-42), visited in insertion order */
         int k0 = c[0] < -1 ? -1 : c[0], k1 = c[1] < -1 ? -1 : c[1], k2 = c[2] < -1 ? -1 : c[2];
         if (k0 > nv[0]) k0 = nv[0];
         if (k1 > nv[1]) k1 = nv[1];
         if (k2 > nv[2]) k2 = nv[2];
         key[3 * a] = k0; key[3 * a + 1] = k1; key[3 * a + 2] = k2;
         int64_t v = ((int64_t)(k0 + 1) * W1 + (k1 + 1)) * W2 + (k2 + 1);
-        next[a] = head[v];
-        head[v] = a;
+        next[a] = -1;
+        if (tail[v] >= 0) next[tail[v]] = a;
+        else head[v] = a;
+        tail[v] = a;
     }
-    free(head); free(next); free(key);
+    free(head); free(tail); free(next); free(key);
     return np;
 }
 

@@ -22,6 +22,8 @@ Fixtures (numpy .npz, no pickled objects):
                  per-term forces and energies, pair count and pair-set digest
   reports.json   sha256 digests of the reference's PDB / TotalEnergy / PVF
                  files for fixed inputs (the byte formats of its Reporter)
+  switch.npz     the LJ switch: single pairs across (0, rc) and a 48-atom
+                 system whose switch values carry over between pairs
   edge.npz       tiny cases: a pair at exactly r = rc, near-collinear
                  torsions, a near-linear angle, atoms outside [lb, ub] at the
                  leapfrog wrap, and a box shorter than 2 rc
@@ -180,6 +182,57 @@ def edge():
     return out
 
 
+def switch():
+    """The LJ switch of DirectForcesAndEnergy (NonbondedForce.cpp:530-585) from
+    the reference itself, switch distance 0.6 nm, cutoff 1.0 nm:
+      single pairs at distances across (0, rc): with one pair per system the
+      switch values carried over from the previous pair play no part, so these
+      pin the polynomial itself;
+      a 48-atom system of charged LJ particles in the reference's pair order:
+      its switch values carry over from pair to pair (switchValue and
+      switchDeriv are assigned only when r > switchingDistance), which the
+      oracle reproduces with switch_quirk = 1."""
+    out = {}
+    rsw, rc = 0.6, 1.0
+    lb, ub = np.zeros(3), np.full(3, 5.0)
+    none2 = (np.zeros(3, np.int32), np.zeros(0, np.int32))
+    rs = np.array([0.35, 0.5, 0.6, 0.61, 0.7, 0.8, 0.9, 0.97, 0.999])
+    q = np.array([0.3, -0.45])
+    sig = np.array([0.3, 0.34])
+    eps = np.array([0.5, 0.8])
+    f_all, e_all = [], []
+    for r in rs:
+        pos = np.array([[2.0, 2.0, 2.0], [2.0 + r * 0.6, 2.0 + r * 0.8, 2.0]])
+        f, es, ed, ex = R.nonbonded_real(pos, q, sig, eps, lb, ub, none2, rc, 5e-4, use_switch=1, rsw=rsw)
+        f_all.append(f)
+        e_all.append([es, ed, ex])
+    out.update(pair_r=rs, pair_q=q, pair_sig=sig, pair_eps=eps, pair_f=np.array(f_all), pair_e=np.array(e_all),
+               rsw=np.float64(rsw), rc=np.float64(rc))
+    # the same pairs uncharged: the direct forces are the switched LJ alone (no
+    # reciprocal part to separate from them)
+    f0_all, e0_all = [], []
+    for r in rs:
+        pos = np.array([[2.0, 2.0, 2.0], [2.0 + r * 0.6, 2.0 + r * 0.8, 2.0]])
+        f, es, ed, ex = R.nonbonded_real(pos, np.zeros(2), sig, eps, lb, ub, none2, rc, 5e-4, use_switch=1, rsw=rsw)
+        f0_all.append(f)
+        e0_all.append([es, ed, ex])
+    out.update(pair0_f=np.array(f0_all), pair0_e=np.array(e0_all))
+    rng = np.random.default_rng(77)
+    n = 48
+    g = np.array([[x, y, z] for x in range(4) for y in range(4) for z in range(3)], float)[:n] * 0.62 + 0.2
+    pos = g + rng.normal(0.0, 0.05, g.shape)
+    qm = rng.normal(0.0, 0.2, n)
+    qm -= qm.mean()
+    sm = rng.uniform(0.28, 0.34, n)
+    em = rng.uniform(0.2, 1.0, n)
+    lbm, ubm = np.zeros(3), np.full(3, 2.6)
+    nonem = (np.zeros(n + 1, np.int32), np.zeros(0, np.int32))
+    f, es, ed, ex = R.nonbonded_real(pos, qm, sm, em, lbm, ubm, nonem, rc, 5e-4, use_switch=1, rsw=rsw)
+    out.update(multi_pos=pos, multi_q=qm, multi_sig=sm, multi_eps=em, multi_lb=lbm, multi_ub=ubm, multi_f=f,
+               multi_e=np.array([es, ed, ex]))
+    return out
+
+
 class _Tiny:
     pass
 
@@ -235,12 +288,16 @@ def reports():
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libnxref.so is not built: run `make -C oracle ref` where the reference exists")
-    for name, fn in (("chain1500", chain1500), ("water3000", water3000), ("edge", edge)):
+    only = sys.argv[1:]        # fixture names to write (default: all)
+    for name, fn in (("chain1500", chain1500), ("water3000", water3000), ("edge", edge), ("switch", switch)):
+        if only and name not in only:
+            continue
         data = {k: np.asarray(v) for k, v in fn().items()}
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **data)
         print("wrote %s (%d arrays, %.0f KB)" % (path, len(data), os.path.getsize(path) / 1024))
-    reports()
+    if not only or "reports" in only:
+        reports()
 
 
 if __name__ == "__main__":

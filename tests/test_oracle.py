@@ -249,3 +249,70 @@ def test_live_moduli_odd_and_smooth_sizes():
     for K in (20, 24, 30, 36, 45, 48, 96):
         mods, _, _, _, _ = R.bspline(np.zeros((1, 3)), np.zeros(3), np.ones(3), (K, K, K))
         np.testing.assert_allclose(O.bspline_moduli(K), mods[0], rtol=1e-13, atol=0)
+
+
+@pytest.fixture(scope="module")
+def sw():
+    return golden("switch")
+
+
+def _switch_pair_system(sw, k):
+    r = float(sw["pair_r"][k])
+    pos = np.array([[2.0, 2.0, 2.0], [2.0 + r * 0.6, 2.0 + r * 0.8, 2.0]])
+    return bare_system(pos, [0, 0, 0], [5.0, 5.0, 5.0], q=sw["pair_q"], sig=sw["pair_sig"], eps=sw["pair_eps"],
+                       cutoff=float(sw["rc"]), grid=(48, 48, 48))
+
+
+def test_switch_single_pairs_vs_reference(sw):
+    """The switch polynomial and its derivative (NonbondedForce.cpp:553-557,
+    :570-573) on one pair at a time, inside, at and beyond the switch distance,
+    against the reference's DirectForcesAndEnergy: with one pair the values
+    carried from a previous pair play no part."""
+    none = (np.zeros(3, np.int32), np.zeros(0, np.int32))
+    for k in range(len(sw["pair_r"])):
+        s = _switch_pair_system(sw, k)
+        f, e = O.nonbonded_real(O.Prepared(s, excl=none, use_switch=1, switch_distance=float(sw["rsw"])))
+        assert rel_rms(f, sw["pair_f"][k]) <= TIGHT, k
+        es, ed, _ = sw["pair_e"][k]
+        assert abs((e["coulomb"] + e["lj"]) - ed) <= TIGHT * max(1.0, abs(ed)), k
+        assert abs(e["self"] - es) <= TIGHT * abs(es)
+    # the switch changes the far pairs and leaves the one inside r_switch alone
+    s = _switch_pair_system(sw, len(sw["pair_r"]) - 2)
+    f0, _ = O.nonbonded_real(O.Prepared(s, excl=none))
+    assert rel_rms(f0, sw["pair_f"][-2]) > 1e-3
+    s = _switch_pair_system(sw, 0)
+    f0, _ = O.nonbonded_real(O.Prepared(s, excl=none))
+    assert rel_rms(f0, sw["pair_f"][0]) <= TIGHT
+
+
+def test_switch_carry_over_vs_reference(sw):
+    """In a many-pair system the reference keeps switchValue / switchDeriv
+    from the last pair beyond the switch distance for the pairs inside it
+    (NonbondedForce.cpp:541, :553-557): the oracle's switch_quirk = 1 gives
+    the reference's forces and energy in the reference's pair order, and the
+    per-pair switch (switch_quirk = 0, what the engine applies) does not."""
+    n = len(sw["multi_q"])
+    s = bare_system(sw["multi_pos"], sw["multi_lb"], sw["multi_ub"], q=sw["multi_q"], sig=sw["multi_sig"],
+                    eps=sw["multi_eps"], cutoff=float(sw["rc"]), grid=(32, 32, 32))
+    none = (np.zeros(n + 1, np.int32), np.zeros(0, np.int32))
+    rsw = float(sw["rsw"])
+    fq, eq = O.nonbonded_real(O.Prepared(s, excl=none, use_switch=1, switch_distance=rsw, switch_quirk=1))
+    assert rel_rms(fq, sw["multi_f"]) <= TIGHT
+    ed = float(sw["multi_e"][1])
+    assert abs((eq["coulomb"] + eq["lj"]) - ed) <= TIGHT * abs(ed)
+    fp, _ = O.nonbonded_real(O.Prepared(s, excl=none, use_switch=1, switch_distance=rsw, switch_quirk=0))
+    assert rel_rms(fp, sw["multi_f"]) > 1e-4
+
+
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (reference sources absent)")
+def test_voxel_pair_order_is_the_references(sw, c0):
+    """The oracle's voxel list emits pairs in the reference's order, not only
+    its set: atoms are visited in insertion order within a voxel, as the
+    reference's push_back vectors are (NeighborList.cpp:37-42, :96-100)."""
+    n = len(sw["multi_q"])
+    none = (np.zeros(n + 1, np.int32), np.zeros(0, np.int32))
+    s = bare_system(sw["multi_pos"], sw["multi_lb"], sw["multi_ub"], cutoff=1.0)
+    assert np.array_equal(O.pairs(s, none, method="voxel"), R.pairs(s.pos, s.lb, s.ub, none, 1.0))
+    c = system_from(c0)
+    ex = O.exclusions(c)
+    assert np.array_equal(O.pairs(c, ex, method="voxel"), R.pairs(c.pos, c.lb, c.ub, ex, c.cutoff))
