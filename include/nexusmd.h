@@ -22,6 +22,7 @@
 #ifndef NEXUSMD_H
 #define NEXUSMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -93,6 +94,8 @@ typedef struct nx_ctx nx_ctx;
 /* Context lifetime.  Replaces Engine::Engine (Engine.cpp:19-50). */
 nx_status nx_create(int device, int precision, nx_ctx** out);
 nx_status nx_destroy(nx_ctx* ctx);
+/* Text of the context's last error; with NULL, the calling thread's last error
+ * of a context-free call (nx_host_alloc, nx_host_free). */
 const char* nx_last_error(const nx_ctx* ctx);
 const char* nx_status_string(nx_status s);
 
@@ -111,6 +114,18 @@ nx_status nx_box_from_positions(int n, const double* pos, double lb[3], double u
 nx_status nx_set_state(nx_ctx* ctx, const double* pos, const double* vel);
 nx_status nx_get_state(nx_ctx* ctx, double* pos, double* vel, double* forces);
 
+/* Neighbour-list state for checkpoints (no counterpart in the reference, which
+ * rebuilds its list every step and keeps no state to restart from): the
+ * positions the current device list was built from (AoS double[n][3]) and
+ * whether a rebuild is due.  A context given a checkpoint's positions,
+ * velocities and list state builds its list from the same positions as the
+ * checkpointed run did, so the mixed path continues bit for bit.
+ * nx_set_list_state must follow nx_set_state (which resets the list state);
+ * the context is laid out again on the next call.  build_pos may be NULL in
+ * nx_get_list_state. */
+nx_status nx_get_list_state(nx_ctx* ctx, double* build_pos, int* rebuild_due);
+nx_status nx_set_list_state(nx_ctx* ctx, const double* build_pos, int rebuild_due);
+
 /* Forces and energies at the current positions (Engine::CalculateForces,
  * Engine.cpp:75-95).  forces may be NULL; e may be NULL. */
 nx_status nx_compute_forces(nx_ctx* ctx, int force_mask, double* forces, nx_energies* e);
@@ -125,6 +140,26 @@ nx_status nx_step(nx_ctx* ctx, int nsteps, double dt, int force_mask);
  * v(t-dt/2), F(t) copied out before the leapfrog update.  Outputs may be NULL. */
 nx_status nx_step_report(nx_ctx* ctx, double dt, int force_mask, double* pos, double* vel, double* forces,
                          nx_energies* e);
+
+/* nx_step_report without waiting (no counterpart in the reference, whose
+ * reports stop its loop, Engine.cpp:117-158): enqueues the same report step
+ * -- forces and energies at x(t), the half-step-shifted kinetic energy, x(t),
+ * v(t-dt/2) and F(t) copied out, then the leapfrog update -- and returns at
+ * once with a ticket.  The copies run on a stream of their own after the
+ * report's kernels, so the update and later steps do not wait for them.  The
+ * output buffers (any may be NULL) must stay valid and untouched until
+ * nx_report_wait(ticket) returns; pinned host memory (nx_host_alloc) makes the
+ * copies asynchronous, pageable memory makes this call wait for them.  At most
+ * 4 reports may be in flight per context (NX_ERR_STATE beyond).  A re-layout
+ * of the context (nx_set_topology, nx_set_box, nx_set_list_state) drops the
+ * reports in flight. */
+nx_status nx_step_report_async(nx_ctx* ctx, double dt, int force_mask, double* pos, double* vel, double* forces,
+                               int64_t* ticket);
+/* Waits until a ticket's outputs are in place; fills its energies (e may be NULL). */
+nx_status nx_report_wait(nx_ctx* ctx, int64_t ticket, nx_energies* e);
+/* Pinned (page-locked) host memory for the outputs of nx_step_report_async. */
+nx_status nx_host_alloc(size_t bytes, void** out);
+nx_status nx_host_free(void* p);
 
 /* The explicit pair set of the current positions: (i, j), i > j, r^2 < rc^2,
  * not excluded -- the set obtainNeighborList emits (NeighborList.cpp:129-154).
@@ -151,6 +186,27 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
 enum { NX_TIMING_OFF = 0, NX_TIMING_KERNELS = 1, NX_TIMING_GRAPH_PAIR = 2 };
 nx_status nx_enable_timing(nx_ctx* ctx, int mode);
 nx_status nx_kernel_time(nx_ctx* ctx, const char* name, double* ms_per_launch, int64_t* launches);
+
+/* Launch shapes (no counterpart in the reference; its FFTW plan choice,
+ * NonbondedForce.cpp:264-290, is the nearest analogue).  0 selects the
+ * built-in rule.  No shape changes what any kernel computes for an atom, a
+ * pair or a grid line, so every choice gives the same bits in the mixed path:
+ *   pair_per_cu    persistent pair-kernel workgroups per CU (capped by occupancy)
+ *   spread_block   PME spread tile side, in neighbour-list columns
+ *   fft_xblocks    plane FFT: number of workgroups the x pass is split into
+ *   fft_lines      line FFT: lines per workgroup */
+typedef struct { int pair_per_cu, spread_block, fft_xblocks, fft_lines; } nx_tuning;
+/* Shapes for the next layout of the context (taken at its next call). */
+nx_status nx_set_tuning(nx_ctx* ctx, const nx_tuning* t);
+nx_status nx_get_tuning(nx_ctx* ctx, nx_tuning* t);
+/* Autotuner: times steps_per_trial steps (>= 4) for each candidate of each
+ * shape in turn (coordinate descent), from the context's current state,
+ * which is restored after every trial; keeps the fastest, applies it and
+ * caches it under the system's key (precision, CU count, atoms, box, grid,
+ * cutoff, buffer) for every later context of the process with that key --
+ * and across processes when NEXUSMD_TUNE_CACHE names a file.  Reports the
+ * chosen shapes and ms per step of the built-in shapes and of the choice. */
+nx_status nx_autotune(nx_ctx* ctx, int steps_per_trial, nx_tuning* chosen, double* ms_default, double* ms_best);
 
 /* Counters of the last call: pairs evaluated by the pair kernel (cluster
  * tiles x 64), pair-list entries, neighbour-list rebuilds. */

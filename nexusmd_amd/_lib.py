@@ -63,6 +63,13 @@ class Stats(C.Structure):
                 ("grid", C.c_int * 3), ("alpha", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pair_per_cu", C.c_int), ("spread_block", C.c_int), ("fft_xblocks", C.c_int), ("fft_lines", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("finished", C.c_int64), ("failed", C.c_int64),
                 ("active", C.c_int64), ("p50_step_ms", C.c_double), ("p99_step_ms", C.c_double),
@@ -81,13 +88,22 @@ SIGNATURES = [
     ("nx_box_from_positions", C.c_int, [C.c_int, D, D, D]),
     ("nx_set_state", C.c_int, [C.c_void_p, D, D]),
     ("nx_get_state", C.c_int, [C.c_void_p, D, D, D]),
+    ("nx_get_list_state", C.c_int, [C.c_void_p, D, I]),
+    ("nx_set_list_state", C.c_int, [C.c_void_p, D, C.c_int]),
     ("nx_compute_forces", C.c_int, [C.c_void_p, C.c_int, D, C.POINTER(Energies)]),
     ("nx_step", C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int]),
     ("nx_step_report", C.c_int, [C.c_void_p, C.c_double, C.c_int, D, D, D, C.POINTER(Energies)]),
+    ("nx_step_report_async", C.c_int, [C.c_void_p, C.c_double, C.c_int, D, D, D, I64]),
+    ("nx_report_wait", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Energies)]),
+    ("nx_host_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("nx_host_free", C.c_int, [C.c_void_p]),
     ("nx_neighbor_pairs", C.c_int, [C.c_void_p, I, C.c_int64, I64]),
     ("nx_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("nx_kernel_time", C.c_int, [C.c_void_p, C.c_char_p, D, I64]),
     ("nx_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("nx_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("nx_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("nx_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning), D, D]),
     ("nx_fft3d_r2c", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D]),
     ("nx_fft3d_c2r", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D]),
     ("nx_server_create", C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),

@@ -74,13 +74,24 @@ __device__ __forceinline__ double angle_term(const SysDev& d, const double* X, c
                                              int4 ijk, double2 pk, V3<real>& f1, V3<real>& f2, V3<real>& f3) {
     const V3<real> v0 = mi<real>(d, X, Y, Z, ijk.y, ijk.x);   // p2 - p1
     const V3<real> v1 = mi<real>(d, X, Y, Z, ijk.y, ijk.z);   // p2 - p3
-    const real r21 = sqrt(dot(v0, v0)), r23 = sqrt(dot(v1, v1));
-    real c = dot(v0, v1) / (r21 * r23);
-    c = c < (real)-1 ? (real)-1 : (c > (real)1 ? (real)1 : c);
-    const real theta = acos(c);
-    const real dI = theta - (real)pk.x;
-    const real dE = (real)pk.y * dI;
     const V3<real> cp = cross(v0, v1);
+    real dI;
+    if constexpr (sizeof(real) == 8) {
+        // the reference's arithmetic: acos of the clamped cosine
+        const real r21 = sqrt(dot(v0, v0)), r23 = sqrt(dot(v1, v1));
+        real c = dot(v0, v1) / (r21 * r23);
+        c = c < (real)-1 ? (real)-1 : (c > (real)1 ? (real)1 : c);
+        dI = acos(c) - (real)pk.x;
+    } else {
+        // FP32: the angle from atan2(|v0 x v1|, v0 . v1), accurate at every angle
+        // (acos of a cosine near +-1 loses half the digits); an obtuse angle is
+        // measured from pi, so that theta - theta0 near a straight angle is the
+        // difference of two small, accurate numbers
+        const real sn = sqrt(dot(cp, cp)), cs = dot(v0, v1);
+        if (cs < 0) dI = (real)(kPi - pk.x) - atan2(sn, -cs);
+        else dI = atan2(sn, cs) - (real)pk.x;
+    }
+    const real dE = (real)pk.y * dI;
     real rp = sqrt(dot(cp, cp));
     rp = rp < (real)1.0e-6 ? (real)1.0e-6 : rp;
     const real q21 = dot(v0, v0), q23 = dot(v1, v1);
@@ -102,18 +113,26 @@ __device__ __forceinline__ double torsion_term(const SysDev& d, const double* X,
     const V3<real> v1 = mi<real>(d, X, Y, Z, q.z, q.y);   // r3 - r2
     const V3<real> v2 = mi<real>(d, X, Y, Z, q.z, q.w);   // r3 - r4
     const V3<real> cp0 = cross(v0, v1), cp1 = cross(v1, v2);
-    const real cosang = dot(unit(cp0), unit(cp1));
     real ang;
-    const real thr = (real)(double)0.99f;   // the reference compares against the float literal 0.99f
-    if (cosang > thr || cosang < -thr) {
-        const V3<real> cr = cross(cp0, cp1);
-        const real scale = dot(cp0, cp0) * dot(cp1, cp1);
-        ang = asin(sqrt(dot(cr, cr) / scale));
-        if (cosang < 0) ang = (real)kPi - ang;
+    if constexpr (sizeof(real) == 8) {
+        // the reference's arithmetic: acos of the cosine, asin where |cos| > 0.99
+        const real cosang = dot(unit(cp0), unit(cp1));
+        const real thr = (real)(double)0.99f;   // the reference compares against the float literal 0.99f
+        if (cosang > thr || cosang < -thr) {
+            const V3<real> cr = cross(cp0, cp1);
+            const real scale = dot(cp0, cp0) * dot(cp1, cp1);
+            ang = asin(sqrt(dot(cr, cr) / scale));
+            if (cosang < 0) ang = (real)kPi - ang;
+        } else {
+            ang = acos(cosang);
+        }
+        ang = (dot(v0, cp1) >= 0 ? ang : -ang);
     } else {
-        ang = acos(cosang);
+        // FP32: phi = atan2(|v1| v0 . cp1, cp0 . cp1), accurate at every angle.  The
+        // same angle and sign: cp0 x cp1 = v1 (v0 . cp1), so |v1| |v0 . cp1| is
+        // |cp0 x cp1|, and the sign is the reference's sign of v0 . cp1
+        ang = atan2(sqrt(dot(v1, v1)) * dot(v0, cp1), dot(cp0, cp1));
     }
-    ang = (dot(v0, cp1) >= 0 ? ang : -ang);
     const real per = (real)pk.z;
     const real delta = per * ang - (real)pk.y;
     const real dEdPhi = -(real)pk.x * per * sin(delta);

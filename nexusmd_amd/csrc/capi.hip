@@ -19,10 +19,15 @@ struct nx_ctx {
     nx::HostSystem sys;
     bool have_top = false, have_box = false, have_state = false, dirty = true;
     int timing = NX_TIMING_OFF;
+    nx::Tuning tuning;            // launch shapes for the next layout (when tuning_set)
+    bool tuning_set = false;
     std::string err;
 };
 
 namespace {
+
+// the calling thread's last error of a context-free entry point (nx_last_error(NULL))
+thread_local std::string tl_error;
 
 struct ArgError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -152,6 +157,7 @@ void ensure(nx_ctx* c) {
     if (c->dirty) {
         if (!c->eng) c->eng.reset(nx::make_engine(c->device, c->precision));
         c->eng->set_timing(c->timing);
+        if (c->tuning_set) c->eng->set_tuning(c->tuning);
         c->eng->setup({&c->sys});
         c->dirty = false;
     }
@@ -199,7 +205,10 @@ nx_status nx_destroy(nx_ctx* c) {
     return NX_OK;
 }
 
-const char* nx_last_error(const nx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* nx_last_error(const nx_ctx* c) {
+    if (c) return c->err.c_str();
+    return tl_error.empty() ? "null context" : tl_error.c_str();
+}
 
 nx_status nx_set_topology(nx_ctx* c, const nx_topology* t) {
     return guarded(c, [&] {
@@ -254,8 +263,44 @@ nx_status nx_set_state(nx_ctx* c, const double* pos, const double* vel) {
         c->sys.pos.assign(pos, pos + 3 * (size_t)n);
         if (vel) c->sys.vel.assign(vel, vel + 3 * (size_t)n);
         else c->sys.vel.assign(3 * (size_t)n, 0.0);
+        c->sys.build_pos.clear();          // new positions: the list is built from them
+        c->sys.rebuild_pending = false;
         c->have_state = true;
         if (!c->dirty && c->eng) c->eng->set_state(pos, vel);
+    });
+}
+
+nx_status nx_get_list_state(nx_ctx* c, double* build_pos, int* rebuild_due) {
+    return guarded(c, [&] {
+        if (c->dirty && c->have_top && c->have_state && !c->sys.build_pos.empty()) {
+            // set by nx_set_list_state and not laid out on the device yet
+            if (build_pos) std::memcpy(build_pos, c->sys.build_pos.data(), c->sys.build_pos.size() * sizeof(double));
+            if (rebuild_due) *rebuild_due = c->sys.rebuild_pending ? 1 : 0;
+            return;
+        }
+        ensure(c);
+        std::vector<double> bp(3 * (size_t)c->sys.n);
+        int due = 0;
+        c->eng->device_list_state(0, bp.data(), &due);
+        if (build_pos) std::memcpy(build_pos, bp.data(), bp.size() * sizeof(double));
+        if (rebuild_due) *rebuild_due = due;
+    });
+}
+
+nx_status nx_set_list_state(nx_ctx* c, const double* build_pos, int rebuild_due) {
+    return guarded(c, [&] {
+        if (!c->have_state) throw ArgError("nx_set_state must precede nx_set_list_state");
+        if (!build_pos) throw ArgError("build positions are NULL");
+        const int n = c->sys.n;
+        check_positions(n, build_pos);
+        if (!c->dirty && c->eng) {
+            // the device state is the current one: keep it as the host copy the
+            // next layout starts from
+            c->eng->device_state(0, c->sys.pos.data(), c->sys.vel.data());
+        }
+        c->sys.build_pos.assign(build_pos, build_pos + 3 * (size_t)n);
+        c->sys.rebuild_pending = rebuild_due != 0;
+        c->dirty = true;                   // laid out again on the next call, the list built from build_pos
     });
 }
 
@@ -320,6 +365,49 @@ nx_status nx_step_report(nx_ctx* c, double dt, int mask, double* pos, double* ve
     });
 }
 
+nx_status nx_step_report_async(nx_ctx* c, double dt, int mask, double* pos, double* vel, double* forces,
+                               int64_t* ticket) {
+    return guarded(c, [&] {
+        if (!(dt > 0) || !std::isfinite(dt)) throw ArgError("dt > 0 required");
+        if (mask & ~NX_FORCE_ALL) throw ArgError("unknown force bits");
+        if (!ticket) throw ArgError("ticket is NULL");
+        ensure(c);
+        *ticket = c->eng->report_async(mask, dt, pos, vel, forces);
+    });
+}
+
+nx_status nx_report_wait(nx_ctx* c, int64_t ticket, nx_energies* e) {
+    return guarded(c, [&] {
+        if (!c->eng || c->dirty) throw ArgError("no report in flight on this context");
+        std::vector<nx_energies> E;
+        with_capacity_recovery(c, [&] { c->eng->report_wait(ticket, e ? &E : nullptr); });
+        if (e) *e = E[0];
+    });
+}
+
+nx_status nx_host_alloc(size_t bytes, void** out) {
+    if (!out) return NX_ERR_ARG;
+    *out = nullptr;
+    if (bytes == 0) return NX_OK;
+    const hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        tl_error = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? NX_ERR_NOMEM : NX_ERR_HIP;
+    }
+    return NX_OK;
+}
+
+nx_status nx_host_free(void* p) {
+    if (!p) return NX_OK;
+    const hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) {
+        tl_error = std::string("hipHostFree: ") + hipGetErrorString(e);
+        return NX_ERR_HIP;
+    }
+    return NX_OK;
+}
+
 nx_status nx_neighbor_pairs(nx_ctx* c, int* pairs, int64_t cap, int64_t* count) {
     return guarded(c, [&] {
         if (cap < 0 || (cap > 0 && !pairs)) throw ArgError("bad pair buffer");
@@ -347,6 +435,59 @@ nx_status nx_kernel_time(nx_ctx* c, const char* name, double* ms, int64_t* launc
         long long n = 0;
         if (!c->eng->kernel_time(name, ms, &n)) { *ms = 0.0; n = 0; }
         *launches = n;
+    });
+}
+
+nx_status nx_set_tuning(nx_ctx* c, const nx_tuning* t) {
+    return guarded(c, [&] {
+        if (!t) throw ArgError("NULL tuning");
+        if (t->pair_per_cu < 0 || t->pair_per_cu > 16 || t->spread_block < 0 || t->spread_block > 64 ||
+            t->fft_xblocks < 0 || t->fft_xblocks > 65536 || t->fft_lines < 0 || t->fft_lines > 8)
+            throw ArgError("tuning value out of range");
+        nx::Tuning u;
+        u.pair_per_cu = t->pair_per_cu;
+        u.spread_block = t->spread_block;
+        u.fft_xblocks = t->fft_xblocks;
+        u.fft_lines = t->fft_lines;
+        c->tuning = u;
+        c->tuning_set = true;
+        if (c->eng && !c->dirty) {
+            // keep the device state across the re-layout, with the list built from the same positions
+            c->eng->device_state(0, c->sys.pos.data(), c->sys.vel.data());
+            c->sys.build_pos.resize(c->sys.pos.size());
+            int due = 0;
+            c->eng->device_list_state(0, c->sys.build_pos.data(), &due);
+            c->sys.rebuild_pending = due != 0;
+        }
+        c->dirty = true;
+    });
+}
+
+nx_status nx_get_tuning(nx_ctx* c, nx_tuning* t) {
+    return guarded(c, [&] {
+        if (!t) throw ArgError("NULL tuning");
+        ensure(c);
+        const nx::Tuning u = c->eng->tuning();
+        t->pair_per_cu = u.pair_per_cu;
+        t->spread_block = u.spread_block;
+        t->fft_xblocks = u.fft_xblocks;
+        t->fft_lines = u.fft_lines;
+    });
+}
+
+nx_status nx_autotune(nx_ctx* c, int steps, nx_tuning* chosen, double* ms_default, double* ms_best) {
+    return guarded(c, [&] {
+        if (steps < 4) throw ArgError("steps_per_trial must be >= 4");
+        ensure(c);
+        const nx::Tuning u = c->eng->autotune(steps, ms_default, ms_best);
+        c->tuning = u;
+        c->tuning_set = true;
+        if (chosen) {
+            chosen->pair_per_cu = u.pair_per_cu;
+            chosen->spread_block = u.spread_block;
+            chosen->fft_xblocks = u.fft_xblocks;
+            chosen->fft_lines = u.fft_lines;
+        }
     });
 }
 

@@ -9,8 +9,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -135,6 +137,60 @@ static int grid_rule(double L) {
     return p;
 }
 
+// ----------------------------------------------------------- tuning cache
+//
+// Launch shapes chosen by autotune, by key, for the process; with
+// NEXUSMD_TUNE_CACHE naming a file, also read from and appended to that file
+// (one "key<TAB>pair spread xblocks lines" line per entry, the last one of a
+// key winning), so that later processes start from them.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, Tuning> g_tune;
+bool g_tune_loaded = false;
+
+void tune_load_locked() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char* path = std::getenv("NEXUSMD_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char line[1024];
+    while (std::fgets(line, sizeof(line), f)) {
+        char* tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        Tuning t;
+        if (std::sscanf(tab + 1, "%d %d %d %d", &t.pair_per_cu, &t.spread_block, &t.fft_xblocks, &t.fft_lines) == 4)
+            g_tune[line] = t;
+    }
+    std::fclose(f);
+}
+}  // namespace
+
+bool tune_cache_get(const std::string& key, Tuning* t) {
+    if (key.empty()) return false;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    *t = it->second;
+    return true;
+}
+
+void tune_cache_put(const std::string& key, const Tuning& t) {
+    if (key.empty()) return;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    g_tune[key] = t;
+    const char* path = std::getenv("NEXUSMD_TUNE_CACHE");
+    if (!path || !*path) return;
+    if (FILE* f = std::fopen(path, "a")) {
+        std::fprintf(f, "%s\t%d %d %d %d\n", key.c_str(), t.pair_per_cu, t.spread_block, t.fft_xblocks, t.fft_lines);
+        std::fclose(f);
+    }
+}
+
 // ----------------------------------------------------------------- engine
 
 template <class P>
@@ -167,9 +223,12 @@ class Engine : public EngineBase {
         if (join_) hipEventDestroy(join_);
         for (auto& e : evpool_) hipEventDestroy(e);
         for (auto& e : step_ev_) hipEventDestroy(e);
+        if (copy_) hipStreamDestroy(copy_);
+        if (packed_) hipEventDestroy(packed_);
     }
 
     void release() {
+        free_reports();
         if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
         if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
         destroy_segments();
@@ -186,6 +245,11 @@ class Engine : public EngineBase {
         hipSetDevice(device_);
         release();
         hs_ = systems;
+        if (!tuning_set_) {
+            // launch shapes an earlier autotune chose for the same kind of system
+            Tuning t;
+            tuning_ = tune_cache_get(tune_key(), &t) ? t : Tuning{};
+        }
         const int ns = (int)systems.size();
         B.nsys = ns;
         HIPCHK(hipDeviceGetAttribute(&B.n_cu, hipDeviceAttributeMultiprocessorCount, device_));
@@ -272,7 +336,11 @@ class Engine : public EngineBase {
                 d.sp_margin = 0.5 * skin + 0.002;
                 long long best = -1;
                 int bb = 0, bnz = 0;
-                for (int b = 1; b <= std::min(12, std::max(ncx, ncy)); ++b) {
+                const int bmax = std::min(12, std::max(ncx, ncy));
+                // a tuned block size is taken when it fits; otherwise the cost model chooses
+                const int want = tuning_.spread_block > 0 ? std::min(tuning_.spread_block, bmax) : 0;
+                for (int pass = want > 0 ? 0 : 1; pass < 2 && best < 0; ++pass)
+                for (int b = pass == 0 ? want : 1; b <= (pass == 0 ? want : bmax); ++b) {
                     for (int nz = 1; nz <= g[2]; ++nz) {
                         auto ext = [&](double w, int K, double Lk, bool full) {
                             if (full) return K;
@@ -368,7 +436,7 @@ class Engine : public EngineBase {
         dalloc(&B.chunks, B.n_chunk_cap, o, B.stream);
         dalloc(&B.chunk_prefix, B.nsys + 1, o, B.stream);
         dalloc(&B.chunk_cost, B.n_chunk_cap, o, B.stream);
-        B.pair_blocks = pair_grid_blocks<P>(B.n_cu, B.n_chunk_cap);
+        B.pair_blocks = pair_grid_blocks<P>(B.n_cu, B.n_chunk_cap, tuning_.pair_per_cu);
         B.pair_waves = B.pair_blocks * kPairWaves;
         dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o, B.stream);
         dalloc(&B.pair_pool, (size_t)kPoolInts, o, B.stream);
@@ -402,7 +470,8 @@ class Engine : public EngineBase {
         int direct_slots = 0;
         const int maxK = fft_maxK_;
         // LDS per FFT block: two buffers of T lines plus the twiddles, within 60 KiB
-        const int T = std::max(1, std::min(8, (int)((61440 / (int)sizeof(real2) - maxK) / (2 * maxK))));
+        const int Tfit = std::max(1, std::min(8, (int)((61440 / (int)sizeof(real2) - maxK) / (2 * maxK))));
+        const int T = tuning_.fft_lines > 0 ? std::min(tuning_.fft_lines, Tfit) : Tfit;
         B.fft_lines = T;
         B.fft_tile = T;
         B.fft_maxK = maxK;
@@ -417,7 +486,8 @@ class Engine : public EngineBase {
                 plane = false;
             sum_ky += d.grid[1];
         }
-        const int xsplit = std::max(1, (512 + sum_ky - 1) / std::max(1, sum_ky));
+        const int xtarget = tuning_.fft_xblocks > 0 ? tuning_.fft_xblocks : 512;
+        const int xsplit = std::max(1, (xtarget + sum_ky - 1) / std::max(1, sum_ky));
         B.fft_plane = plane ? 1 : 0;
         B.fft_plane_lds = 0;
         B.fft_x_lds = 0;
@@ -960,6 +1030,11 @@ class Engine : public EngineBase {
     void energies_out(int s, int mask, nx_energies* e) {
         std::vector<long long> h(E_NTERMS);
         copy_sync(h.data(), B.energy + (size_t)s * E_NTERMS, sizeof(long long) * E_NTERMS, hipMemcpyDeviceToHost);
+        energies_from(h.data(), s, mask, e);
+    }
+
+    // system s's energies from its E_NTERMS fixed-point sums h
+    void energies_from(const long long* h, int s, int mask, nx_energies* e) {
         auto v = [&](int t) { return (double)h[t] / kEnergyScale; };
         std::memset(e, 0, sizeof(*e));
         e->bond = (mask & 1) ? v(E_BOND) : 0.0;
@@ -1001,6 +1076,109 @@ class Engine : public EngineBase {
             pack_needed_ = true;          // the slot forces still hold this evaluation
         }
         end_call();
+    }
+
+    // ------------------------------------------------------ async reports
+    //
+    // One report step without a host round trip: forces and energies at x(t),
+    // the kinetic energy with the half-step shift, a snapshot of x(t),
+    // v(t - dt/2) and F(t) into a device staging block of the slot, then the
+    // leapfrog update, all on the engine stream.  A copy stream moves the
+    // snapshot, the energy sums and the list counters to the caller's (pinned)
+    // buffers after an event, so the update and the next steps do not wait for
+    // the copies.  report_wait(ticket) waits for the slot's copies and returns
+    // its energies.  A slot's staging is reused only after its ticket was
+    // waited for, so no copy can read a block that a later report writes.
+    long long report_async(int mask, double dt, double* pos, double* vel, double* F) override {
+        int k = -1;
+        for (int i = 0; i < kReportSlots; ++i)
+            if (rs_[i].ticket < 0) { k = i; break; }
+        if (k < 0)
+            throw HipError(NX_ERR_STATE, "too many reports in flight (" + std::to_string(kReportSlots) +
+                                             "): wait for one with nx_report_wait first");
+        ReportSlot& r = rs_[k];
+        const size_t n = B.n_atoms;
+        const size_t ebytes = sizeof(long long) * B.nsys * E_NTERMS, cbytes = sizeof(SysCount) * B.nsys;
+        if (!copy_) HIPCHK(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking));
+        if (!packed_) HIPCHK(hipEventCreateWithFlags(&packed_, hipEventDisableTiming));
+        if (!r.d_stage) {
+            HIPCHK(hipMalloc(&r.d_stage, 9 * n * sizeof(double)));
+            HIPCHK(hipMalloc(&r.d_energy, ebytes));
+            HIPCHK(hipMalloc(&r.d_cnt, cbytes));
+            HIPCHK(hipHostMalloc(&r.h_energy, ebytes, hipHostMallocDefault));
+            HIPCHK(hipHostMalloc(&r.h_cnt, cbytes, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+        }
+        evs_.clear();
+        ev_used_ = 0;
+        HIPCHK(hipGetLastError());
+        settle();
+        zero_energy();
+        forces(mask, true);
+        launch_kinetic<P>(B, dt);
+        launch_report_snapshot<P>(B, r.d_stage);
+        HIPCHK(hipMemcpyAsync(r.d_energy, B.energy, ebytes, hipMemcpyDeviceToDevice, B.stream));
+        HIPCHK(hipMemcpyAsync(r.d_cnt, B.d_cnt, cbytes, hipMemcpyDeviceToDevice, B.stream));
+        HIPCHK(hipEventRecord(packed_, B.stream));
+        HIPCHK(hipStreamWaitEvent(copy_, packed_, 0));
+        double* dst[3] = {pos, vel, F};
+        for (int b = 0; b < 3; ++b)
+            if (dst[b])
+                HIPCHK(hipMemcpyAsync(dst[b], r.d_stage + 3 * n * b, 3 * n * sizeof(double), hipMemcpyDeviceToHost,
+                                      copy_));
+        HIPCHK(hipMemcpyAsync(r.h_energy, r.d_energy, ebytes, hipMemcpyDeviceToHost, copy_));
+        HIPCHK(hipMemcpyAsync(r.h_cnt, r.d_cnt, cbytes, hipMemcpyDeviceToHost, copy_));
+        HIPCHK(hipEventRecord(r.done, copy_));
+        launch_integrate<P>(B, dt);
+        ++steps_;
+        pack_needed_ = false;
+        r.mask = mask;
+        r.ticket = next_ticket_++;
+        return r.ticket;
+    }
+
+    void report_wait(long long ticket, std::vector<nx_energies>* E) override {
+        int k = -1;
+        for (int i = 0; i < kReportSlots; ++i)
+            if (rs_[i].ticket == ticket && ticket >= 0) k = i;
+        if (k < 0) throw HipError(NX_ERR_ARG, "unknown or already collected report ticket " + std::to_string(ticket));
+        ReportSlot& r = rs_[k];
+        const hipError_t e = hipEventSynchronize(r.done);
+        r.ticket = -1;
+        HIPCHK(e);
+        HIPCHK(hipGetLastError());
+        counts_.assign(r.h_cnt, r.h_cnt + B.nsys);
+        failed_sys_.clear();
+        for (int s = 0; s < B.nsys; ++s)
+            if (r.h_cnt[s].overflow) failed_sys_.push_back(s);
+        if (!failed_sys_.empty())
+            throw HipError(NX_ERR_CAPACITY, "neighbour-list capacity exceeded in system " +
+                                                std::to_string(failed_sys_[0]) + " (code " +
+                                                std::to_string(r.h_cnt[failed_sys_[0]].overflow) + ")");
+        if (E) {
+            E->resize(B.nsys);
+            for (int s = 0; s < B.nsys; ++s) energies_from(r.h_energy + (size_t)s * E_NTERMS, s, r.mask, &(*E)[s]);
+        }
+    }
+
+    // waits for every report in flight (their results are dropped) and frees the slots
+    void drain_reports() {
+        for (auto& r : rs_) {
+            if (r.done && r.ticket >= 0) hipEventSynchronize(r.done);
+            r.ticket = -1;
+        }
+    }
+    void free_reports() {
+        drain_reports();
+        for (auto& r : rs_) {
+            if (r.d_stage) hipFree(r.d_stage);
+            if (r.d_energy) hipFree(r.d_energy);
+            if (r.d_cnt) hipFree(r.d_cnt);
+            if (r.h_energy) hipHostFree(r.h_energy);
+            if (r.h_cnt) hipHostFree(r.h_cnt);
+            if (r.done) hipEventDestroy(r.done);
+            r = ReportSlot{};
+        }
     }
 
     void gather_forces(std::vector<double>& F) {
@@ -1122,6 +1300,98 @@ class Engine : public EngineBase {
         return true;
     }
 
+    // ------------------------------------------------------------ tuning
+
+    void set_tuning(const Tuning& t) override {
+        tuning_ = t;
+        tuning_set_ = t.pair_per_cu || t.spread_block || t.fft_xblocks || t.fft_lines;
+    }
+    Tuning tuning() const override { return tuning_; }
+
+    // What the launch shapes depend on: precision, the device's CU count and, per
+    // system, its atom count, box (to 0.05 nm), PME grid and cutoff.  Batches of
+    // more than four systems are not cached.
+    std::string tune_key() const {
+        if (hs_.empty() || hs_.size() > 4) return std::string();
+        char buf[160];
+        int cu = 0;
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device_);
+        std::string k = std::string(P::mixed ? "mixed" : "fp64") + "|cu" + std::to_string(cu);
+        for (const HostSystem* h : hs_) {
+            int g[3];
+            for (int a = 0; a < 3; ++a) g[a] = h->nb.grid[a] > 0 ? h->nb.grid[a] : grid_rule(fabs(h->ub[a] - h->lb[a]));
+            std::snprintf(buf, sizeof(buf), "|n%d,L%.0f,%.0f,%.0f,g%dx%dx%d,rc%.3f,sk%.3f", h->n,
+                          20.0 * fabs(h->ub[0] - h->lb[0]), 20.0 * fabs(h->ub[1] - h->lb[1]),
+                          20.0 * fabs(h->ub[2] - h->lb[2]), g[0], g[1], g[2], h->nb.cutoff, h->nb.skin);
+            k += buf;
+        }
+        return k;
+    }
+
+    Tuning autotune(int steps, double* ms_default, double* ms_best) override {
+        if (B.nsys == 0 || hs_.empty()) throw HipError(NX_ERR_STATE, "nothing laid out to tune");
+        if (steps < 4) throw HipError(NX_ERR_ARG, "autotune needs at least 4 steps per trial");
+        const int mask = NX_FORCE_ALL;
+        const double dt = 0.0005;
+        // every trial lays the batch out again from the current state, with the
+        // list built from the same positions, and the state is restored after
+        for (int s = 0; s < B.nsys; ++s) {
+            HostSystem& h = *hs_[s];
+            device_state(s, h.pos.data(), h.vel.data());
+            h.build_pos.resize(h.pos.size());
+            int due = 0;
+            device_list_state(s, h.build_pos.data(), &due);
+            h.rebuild_pending = due != 0;
+        }
+        const std::vector<HostSystem*> systems = hs_;
+        auto trial = [&](const Tuning& t) {
+            tuning_ = t;
+            tuning_set_ = true;
+            setup(systems);
+            run(4, dt, mask);                        // graph capture and warm-up
+            const auto t0 = std::chrono::steady_clock::now();
+            run(steps, dt, mask);
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            return ms / steps;
+        };
+        Tuning best{};
+        double best_ms = trial(best);
+        auto consider = [&](Tuning t) {
+            const double ms = trial(t);
+            if (ms < 0.99 * best_ms) { best_ms = ms; best = t; }
+        };
+        const int occ = std::min(5, pair_blocks_occupancy<P>());
+        {
+            const Tuning base = best;
+            for (int c = 1; c <= occ; ++c) { Tuning t = base; t.pair_per_cu = c; consider(t); }
+        }
+        {
+            int bmax = 1;
+            for (const SysDev& d : B.sys) bmax = std::max(bmax, std::min(12, std::max(d.ncx, d.ncy)));
+            const Tuning base = best;
+            for (int b = 1; b <= bmax; ++b) { Tuning t = base; t.spread_block = b; consider(t); }
+        }
+        {
+            const Tuning base = best;
+            if (B.fft_plane) {
+                for (int x : {128, 256, 1024, 2048}) { Tuning t = base; t.fft_xblocks = x; consider(t); }
+            } else {
+                for (int l : {1, 2, 4}) { Tuning t = base; t.fft_lines = l; consider(t); }
+            }
+        }
+        // the two finalists measured once more, side by side
+        const double d_ms = trial(Tuning{});
+        const double b_ms = trial(best);
+        if (b_ms > d_ms) best = Tuning{};
+        if (ms_default) *ms_default = d_ms;
+        if (ms_best) *ms_best = std::min(b_ms, d_ms);
+        tuning_ = best;
+        tuning_set_ = true;
+        setup(systems);                              // the state as it was, with the chosen shapes
+        tune_cache_put(tune_key(), best);
+        return best;
+    }
+
     void set_timing(int mode) override { B.timing = mode; }
     void set_graphs(bool on) override { graphs_enabled_ = on; }
     int nsys() const override { return B.nsys; }
@@ -1172,6 +1442,24 @@ class Engine : public EngineBase {
     long long* d_count_ = nullptr;
     std::vector<hipEvent_t> evpool_;
     std::vector<hipEvent_t> step_ev_;       // run_step_times: one event per step boundary
+    Tuning tuning_;                         // launch shapes of the current layout
+    bool tuning_set_ = false;               // set explicitly or by autotune (else: from the cache)
+    // async reports: staging on the device, energy sums and counters in pinned memory
+    static constexpr int kReportSlots = 4;
+    struct ReportSlot {
+        double* d_stage = nullptr;          // [3][n][3]: positions, velocities, forces
+        long long* d_energy = nullptr;
+        SysCount* d_cnt = nullptr;
+        long long* h_energy = nullptr;
+        SysCount* h_cnt = nullptr;
+        hipEvent_t done = nullptr;          // the slot's copies have landed
+        long long ticket = -1;              // -1: free
+        int mask = 0;
+    };
+    ReportSlot rs_[kReportSlots];
+    hipStream_t copy_ = nullptr;
+    hipEvent_t packed_ = nullptr;
+    long long next_ticket_ = 1;
     size_t ev_used_ = 0;
     std::vector<Ev> evs_;
     std::map<std::string, std::pair<double, long long>> times_;

@@ -166,6 +166,41 @@ __global__ void collect_kernel(const int* __restrict__ slot_of_atom, const typen
     FZ[a] = f[2];
 }
 
+// A report's snapshot in atom order, AoS, as three blocks [n][3]: positions,
+// velocities, total forces (the layout the reports and the caller take).
+template <class P>
+__global__ void report_snapshot_kernel(const int* __restrict__ slot_of_atom, const double* __restrict__ X,
+                                       const double* __restrict__ Y, const double* __restrict__ Z,
+                                       const double* __restrict__ VX, const double* __restrict__ VY,
+                                       const double* __restrict__ VZ, const typename P::facc* FAx,
+                                       const typename P::facc* FAy, const typename P::facc* FAz,
+                                       const typename P::facc* FSx, const typename P::facc* FSy,
+                                       const typename P::facc* FSz, double* out, int n_atoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_atoms) return;
+    double f[3];
+    total_force<P>(a, slot_of_atom[a], FAx, FAy, FAz, FSx, FSy, FSz, f);
+    const size_t n = (size_t)n_atoms, i = 3 * (size_t)a;
+    out[i] = X[a];
+    out[i + 1] = Y[a];
+    out[i + 2] = Z[a];
+    out[3 * n + i] = VX[a];
+    out[3 * n + i + 1] = VY[a];
+    out[3 * n + i + 2] = VZ[a];
+    out[6 * n + i] = f[0];
+    out[6 * n + i + 1] = f[1];
+    out[6 * n + i + 2] = f[2];
+}
+
+template <class P>
+void launch_report_snapshot(Batch<P>& B, double* out) {
+    const int T = 256;
+    report_snapshot_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
+        B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, out, B.n_atoms);
+}
+template void launch_report_snapshot<PrecFP64>(Batch<PrecFP64>&, double*);
+template void launch_report_snapshot<PrecMixed>(Batch<PrecMixed>&, double*);
+
 template <class P>
 void launch_pack(Batch<P>& B) {
     const int T = 256;

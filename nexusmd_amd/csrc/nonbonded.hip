@@ -42,7 +42,7 @@ struct MixedPair {
     float cg;       // 2 alpha / sqrt(pi)
     float sw_on;    // switch start, < 0 without a switch
     float sw_inv;   // 1 / (rc - r_switch)
-    float sw_b;     // -r_switch / (rc - r_switch): t = r sw_inv + sw_b
+    float sw_rc;    // the cutoff: 1 - t = (rc - r) sw_inv
     float sw_d;     // -30 / (rc - r_switch): dsv/dr = sw_d t^2 (1 - t)^2
 };
 
@@ -53,7 +53,7 @@ __device__ __forceinline__ MixedPair mixed_pair_const(const SysDev& d) {
     c.cg = (float)(kTwoOverSqrtPi * d.alpha);
     c.sw_on = (float)d.sw_on;
     c.sw_inv = (float)d.sw_inv;
-    c.sw_b = (float)(-d.sw_on * d.sw_inv);
+    c.sw_rc = (float)d.rc;
     c.sw_d = (float)(-30.0 * d.sw_inv);
     return c;
 }
@@ -73,10 +73,13 @@ __device__ __forceinline__ float pair_mixed(float r2, float qq, float sig, float
     float le = epsij * (s6 - 1.0f) * s6;
     float dl = epsij * (12.0f * s6 - 6.0f) * s6 * inv_r2;
     if (c.sw_on >= 0.f) {   // uniform per system; t = 0 inside r_switch gives sv = 1, sd = 0
-        // sv = 1 - 10 t^3 + 15 t^4 - 6 t^5, dsv/dt = -30 t^2 (1 - t)^2
-        const float t = fmaxf(fmaf(r, c.sw_inv, c.sw_b), 0.0f);
-        const float sv = fmaf(t * t * t, fmaf(t, fmaf(t, -6.0f, 15.0f), -10.0f), 1.0f);
-        const float u = fmaf(-t, t, t);
+        // sv = 1 - 10 t^3 + 15 t^4 - 6 t^5 = w^3 (1 + 3 t + 6 t^2) with w = 1 - t,
+        // dsv/dt = -30 t^2 w^2.  The factored form keeps FP32 accuracy where sv
+        // is small (r near rc); w = 1 inside r_switch gives sv = 1, sd = 0.
+        const float w = fminf((c.sw_rc - r) * c.sw_inv, 1.0f);
+        const float t = 1.0f - w;
+        const float sv = w * w * w * fmaf(t, fmaf(t, 6.0f, 3.0f), 1.0f);
+        const float u = t * w;
         const float sd = c.sw_d * u * u;
         dl = sv * dl - le * sd * inv_r;
         le *= sv;
@@ -481,8 +484,10 @@ static int pair_blocks_per_cu() {
 // blocks possible): 3 per CU 261 us, 4 per CU 303 us, 2 per CU 268 us.
 // Fixed per batch layout: the list build cuts the chunk ranges for it.
 template <class P>
-int pair_grid_blocks(int n_cu, int n_chunk_cap) {
-    const int per_cu = std::min(pair_blocks_per_cu<P>(), 3);
+int pair_grid_blocks(int n_cu, int n_chunk_cap, int per_cu_request) {
+    // a requested count (the autotuner's) is capped by what a CU can hold at once
+    const int occ = pair_blocks_per_cu<P>();
+    const int per_cu = per_cu_request > 0 ? std::min(per_cu_request, occ) : std::min(occ, 3);
     return std::max(1, std::min((n_chunk_cap + kPairWaves - 1) / kPairWaves, per_cu * std::max(1, n_cu)));
 }
 
@@ -524,8 +529,15 @@ void launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_cou
                                                                B.n_super_cap);
 }
 
-template int pair_grid_blocks<PrecFP64>(int, int);
-template int pair_grid_blocks<PrecMixed>(int, int);
+template int pair_grid_blocks<PrecFP64>(int, int, int);
+template int pair_grid_blocks<PrecMixed>(int, int, int);
+
+template <class P>
+int pair_blocks_occupancy() {
+    return pair_blocks_per_cu<P>();
+}
+template int pair_blocks_occupancy<PrecFP64>();
+template int pair_blocks_occupancy<PrecMixed>();
 template void launch_pair<PrecFP64>(Batch<PrecFP64>&, bool);
 template void launch_pair<PrecMixed>(Batch<PrecMixed>&, bool);
 template void launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);

@@ -132,7 +132,9 @@ struct Batch {
 };
 
 template <class P> void launch_nlist(Batch<P>& B, bool clear_flags);
-template <class P> int pair_grid_blocks(int n_cu, int n_chunk_cap);
+// persistent pair-kernel grid: per_cu_request blocks per CU (0: the built-in count), capped by occupancy
+template <class P> int pair_grid_blocks(int n_cu, int n_chunk_cap, int per_cu_request);
+template <class P> int pair_blocks_occupancy();     // pair-kernel blocks one CU holds at once
 template <class P> void launch_pack(Batch<P>& B);
 template <class P> void launch_pair(Batch<P>& B, bool energy);
 template <class P> void launch_bonded(Batch<P>& B, int mask, bool energy);
@@ -147,9 +149,12 @@ template <class P> void launch_pme_gather(Batch<P>& B);
 template <class P> void launch_kinetic(Batch<P>& B, double dt);
 template <class P> void launch_integrate(Batch<P>& B, double dt);
 template <class P> void launch_collect_forces(Batch<P>& B, double* FX, double* FY, double* FZ);
+// positions, velocities and total forces in atom order into out[3][n][3]
+template <class P> void launch_report_snapshot(Batch<P>& B, double* out);
 // Writes up to cap pairs to out and their total count to *d_count, which must be zero before the launch.
 template <class P> void launch_export_pairs(Batch<P>& B, int2* out, long long cap, long long* d_count);
 // standalone FFT (verification)
 template <class P> void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool forward,
                                         hipStream_t st);
+
 }  // namespace nx

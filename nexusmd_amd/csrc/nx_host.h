@@ -42,6 +42,16 @@ struct HostSystem {
 
 void exclusions_from_bonds(int n, const std::vector<nx_bond>& bonds, std::vector<int>& off, std::vector<int>& idx);
 
+// Launch shapes the autotuner chooses among (0 = the built-in rule).  None of
+// them changes what a kernel computes for a given atom, pair or grid line, so
+// every choice gives the same bits in the mixed path.
+struct Tuning {
+    int pair_per_cu = 0;     // persistent pair-kernel blocks per CU (built-in: min(occupancy, 3))
+    int spread_block = 0;    // spread tile: columns per block side (built-in: the cost model's choice)
+    int fft_xblocks = 0;     // plane FFT: x-pass blocks the kz range is split into (built-in: ~512)
+    int fft_lines = 0;       // line FFT: lines per block (built-in: as many as fit 60 KiB of LDS, <= 8)
+};
+
 class EngineBase {
    public:
     virtual ~EngineBase() = default;
@@ -52,6 +62,10 @@ class EngineBase {
     virtual void compute(int mask, bool kinetic, double dt, bool integrate, std::vector<double>* F,
                          std::vector<double>* pos, std::vector<double>* vel, std::vector<nx_energies>* E) = 0;
     virtual void get_state(std::vector<double>* pos, std::vector<double>* vel) = 0;
+    // one report step enqueued without waiting (see nx_step_report_async); returns its ticket
+    virtual long long report_async(int mask, double dt, double* pos, double* vel, double* F) = 0;
+    // waits for a ticket's copies; energies per system
+    virtual void report_wait(long long ticket, std::vector<nx_energies>* E) = 0;
     virtual void set_state(const double* pos, const double* vel) = 0;
     virtual long long export_pairs(std::vector<int>* out) = 0;   // nullptr: count only
     virtual void stats(nx_stats* st) = 0;
@@ -66,6 +80,14 @@ class EngineBase {
     // build positions (AoS) of system s's current neighbour list and its rebuild flag
     virtual void device_list_state(int s, double* build_pos, int* rebuild_due) = 0;
     virtual size_t bytes() const = 0;
+    // launch shapes: set (a re-layout follows at the next setup), read, or chosen by timed trials
+    virtual void set_tuning(const Tuning& t) = 0;
+    virtual Tuning tuning() const = 0;
+    // Times `steps` steps per candidate, knob by knob, on the current systems and
+    // their current state (restored after every trial and at the end), keeps the
+    // fastest, and caches it under the batch's key.  Returns ms/step of the
+    // built-in shapes and of the choice.
+    virtual Tuning autotune(int steps, double* ms_default, double* ms_best) = 0;
 };
 
 EngineBase* make_engine(int device, int precision);

@@ -113,6 +113,17 @@ class Context:
         self._check(self.lib.nx_get_state(self.h, _d(p), _d(v), _d(f)), "nx_get_state")
         return (p, v, f) if forces else (p, v)
 
+    def get_list_state(self):
+        """(build positions (n, 3), rebuild due) of the device neighbour list."""
+        bp = np.zeros((self.n, 3))
+        due = C.c_int(0)
+        self._check(self.lib.nx_get_list_state(self.h, _d(bp), C.byref(due)), "nx_get_list_state")
+        return bp, int(due.value)
+
+    def set_list_state(self, build_pos, rebuild_due):
+        bp = np.ascontiguousarray(build_pos, np.float64)
+        self._check(self.lib.nx_set_list_state(self.h, _d(bp), int(rebuild_due)), "nx_set_list_state")
+
     def compute_forces(self, mask=L.ALL):
         f = np.zeros((self.n, 3))
         e = L.Energies()
@@ -159,6 +170,23 @@ class Context:
         n = C.c_int64(0)
         self._check(self.lib.nx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)), "nx_kernel_time")
         return ms.value, n.value
+
+    def set_tuning(self, pair_per_cu=0, spread_block=0, fft_xblocks=0, fft_lines=0):
+        t = L.Tuning(int(pair_per_cu), int(spread_block), int(fft_xblocks), int(fft_lines))
+        self._check(self.lib.nx_set_tuning(self.h, C.byref(t)), "nx_set_tuning")
+
+    def get_tuning(self):
+        t = L.Tuning()
+        self._check(self.lib.nx_get_tuning(self.h, C.byref(t)), "nx_get_tuning")
+        return t.as_dict()
+
+    def autotune(self, steps_per_trial=50):
+        """Times launch-shape candidates on this system; returns (chosen, ms_default, ms_best)."""
+        t = L.Tuning()
+        a, b = C.c_double(0), C.c_double(0)
+        self._check(self.lib.nx_autotune(self.h, int(steps_per_trial), C.byref(t), C.byref(a), C.byref(b)),
+                    "nx_autotune")
+        return t.as_dict(), a.value, b.value
 
     def stats(self):
         s = L.Stats()

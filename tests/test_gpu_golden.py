@@ -149,8 +149,9 @@ def test_bonded_edge_cases(edge, prec):
     s.angle_idx, s.angle_par = edge["ang_idx"], edge["ang_par"]
     ctx = E.Context(s, precision=prec, excl=(np.zeros(s.n + 1, np.int32), np.zeros(0, np.int32)))
     f, e = ctx.compute_forces(L.TORSION)
-    # near-collinear geometry: FP32 angles lose digits, hold the mixed path to 5e-3 here
-    tol = 1e-9 if prec == L.FP64 else 5e-3
+    # near-collinear geometry: the mixed path takes its FP32 angles from atan2 (an
+    # obtuse angle measured from pi), so it holds its ordinary bound here too
+    tol = 1e-9 if prec == L.FP64 else 5e-4
     assert rel_rms(f, edge["tors_f"]) <= tol
     f, e = ctx.compute_forces(L.ANGLE)
     assert rel_rms(f, edge["ang_f"]) <= tol
@@ -172,3 +173,32 @@ def test_leapfrog_wrap(edge):
     np.testing.assert_allclose(p, x1, rtol=0, atol=1e-15)
     np.testing.assert_array_equal(v, vel)
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_switch_single_pairs_vs_reference(prec):
+    """The LJ switch on one uncharged pair at a time, across (0, rc) with the
+    switch from 0.6 nm, against the reference's DirectForcesAndEnergy with
+    useSwitchingFunction = 1 (tests/golden/switch.npz): with one pair there is
+    no previous pair whose switch values could carry over, so the reference's
+    switch and the engine's per-pair switch must agree; with no charges the
+    nonbonded force is the switched LJ alone."""
+    sw = golden("switch")
+    none = (np.zeros(3, np.int32), np.zeros(0, np.int32))
+    for k, r in enumerate(sw["pair_r"]):
+        pos = np.array([[2.0, 2.0, 2.0], [2.0 + r * 0.6, 2.0 + r * 0.8, 2.0]])
+        s = bare_system(pos, [0, 0, 0], [5.0, 5.0, 5.0], sig=sw["pair_sig"], eps=sw["pair_eps"],
+                        cutoff=float(sw["rc"]), grid=(48, 48, 48))
+        print("[gpu] switch pair r=%.3f precision=%d" % (r, prec), flush=True)
+        ctx = E.Context(s, precision=prec, skin=0.0, excl=none, use_switch=1, switch_distance=float(sw["rsw"]))
+        f, e = ctx.compute_forces(L.NONBONDED)
+        ctx.close()
+        want = sw["pair0_f"][k]
+        # mixed: relative bound, or, for the vanishing force near the cutoff (about
+        # 1e-6 kJ/mol/nm at r = 0.999 nm), an absolute bound set by the FP32
+        # coordinates: their rounding (~2.4e-7 nm at 2.6 nm) moves 1 - t by ~2e-4 of
+        # itself, and the switched force goes with (1 - t)^2
+        err, absdiff = rel_rms(f, want), float(np.abs(f - want).max())
+        assert err <= TOL[prec] or (prec == L.MIXED and absdiff <= 2e-9), (k, r, err, absdiff)
+        ed = float(sw["pair0_e"][k][1])
+        assert abs(e["lj"] - ed) <= (1e-9 if prec == L.FP64 else 2e-5) * max(1.0, abs(ed)) + (1e-9 if prec == L.FP64 else 1e-4)
