@@ -15,6 +15,7 @@
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <memory>
 #include <string>
 #include <vector>
@@ -23,7 +24,42 @@
 
 namespace nexusmd {
 
-using Coords3D = std::array<double, 3>;
+// Three-component vector with the interface of the reference's Coords3D
+// (core/common/headers/Coords3D.h:15-151): components by index, component-
+// wise + and -, scaling by a double, dot and cross products, length and a
+// normalised copy (the zero vector stays zero).  dot adds x, y, z left to
+// right, as the reference does.
+class Coords3D {
+   public:
+    Coords3D() = default;
+    Coords3D(double x, double y, double z) : data{x, y, z} {}
+    double operator[](int i) const { return data[i]; }
+    double& operator[](int i) { return data[i]; }
+    bool operator==(const Coords3D& o) const { return data[0] == o[0] && data[1] == o[1] && data[2] == o[2]; }
+    bool operator!=(const Coords3D& o) const { return !(*this == o); }
+    Coords3D operator+() const { return *this; }
+    Coords3D operator-() const { return Coords3D(-data[0], -data[1], -data[2]); }
+    Coords3D operator+(const Coords3D& o) const { return Coords3D(data[0] + o[0], data[1] + o[1], data[2] + o[2]); }
+    Coords3D operator-(const Coords3D& o) const { return Coords3D(data[0] - o[0], data[1] - o[1], data[2] - o[2]); }
+    Coords3D& operator+=(const Coords3D& o) { return *this = *this + o; }
+    Coords3D& operator-=(const Coords3D& o) { return *this = *this - o; }
+    Coords3D operator*(double s) const { return Coords3D(data[0] * s, data[1] * s, data[2] * s); }
+    Coords3D& operator*=(double s) { return *this = *this * s; }
+    // division multiplies by the reciprocal, as the reference does
+    Coords3D operator/(double s) const { return *this * (1.0 / s); }
+    Coords3D& operator/=(double s) { return *this = *this / s; }
+    double dot(const Coords3D& o) const { return data[0] * o[0] + data[1] * o[1] + data[2] * o[2]; }
+    Coords3D cross(const Coords3D& o) const {
+        return Coords3D(data[1] * o[2] - data[2] * o[1], data[2] * o[0] - data[0] * o[2], data[0] * o[1] - data[1] * o[0]);
+    }
+    double length() const { return std::sqrt(dot(*this)); }
+    // each component divided by the length
+    Coords3D normalize() const {
+        const double l = length();
+        return l > 0.0 ? Coords3D(data[0] / l, data[1] / l, data[2] / l) : Coords3D();
+    }
+    double data[3] = {0.0, 0.0, 0.0};
+};
 
 using BondParams = nx_bond;          // p1, p2, d, k          (SystemXMLParser.h:16-21)
 using AngleParams = nx_angle;        // p1, p2, p3, a, k      (SystemXMLParser.h:23-27)
@@ -66,10 +102,34 @@ class SystemXMLParser {
     static NonbondedParams NonBondedParser(const std::string& filename);
 };
 
+// Everything a state file carries: the OpenMM-style <State> (positions; when
+// present velocities, box vectors, step count and time) and this engine's
+// restart extension -- the exact box bounds and the neighbour-list state (the
+// positions the current list was built from and whether a rebuild is due),
+// without which a restarted mixed-precision run would not continue bit for bit.
+struct StateData {
+    std::vector<Coords3D> positions, velocities, buildPositions;
+    bool hasVelocities = false, hasBoxVectors = false, hasBoxBounds = false, hasListState = false;
+    Coords3D boxA, boxB, boxC;       // <PeriodicBoxVectors> A, B, C (nm)
+    Coords3D lb, ub;                 // <NexusMDBox>: box bounds as the engine uses them
+    int rebuildDue = 0;
+    long long stepCount = 0;
+    double time = 0.0;               // ps
+};
+
 class StateXMLParser {
    public:
     // <State><Positions><Position x y z> in nm (StateXMLParser.cpp:17-37)
     static std::vector<Coords3D> parseStateXML(const std::string& filename);
+    // (A.x, B.y, C.z) of <PeriodicBoxVectors> (StateXMLParser.cpp:39-60; the
+    // reference defines it and never calls it)
+    static Coords3D extractBoxSize(const std::string& filename);
+    // Every section of StateData in one pass over the file; throws
+    // std::runtime_error when the file cannot be read or parsed.
+    static StateData parseState(const std::string& filename);
+    // A restartable state file: every section of StateData that is present,
+    // doubles written with 17 significant digits (they read back exactly).
+    static void writeState(const std::string& filename, const StateData& state);
 };
 
 struct PDBAtom {
@@ -127,10 +187,32 @@ class TaskDispatcher {
     void EnablePeriodicTorsionForce() { periodicTorsionForceEnabled = true; }
     void EnableNonbondedForce() { nonbondedForceEnabled = true; }
 
+    // State I/O beyond the reference (each off by default, which keeps the
+    // reference's behaviour: v(0) = 0, Initializer.cpp:21; the box from the
+    // positions' bounding box, PeriodicBoundaryCondition.cpp:11-88, the state's
+    // PeriodicBoxVectors unused, Engine.cpp:45-48).
+    // v(0) from the state's <Velocities>
+    void EnableStateVelocities() { useStateVelocities = true; }
+    // the box from the state's <PeriodicBoxVectors> (rectangular; origin 0)
+    void EnableStateBoxVectors() { useStateBoxVectors = true; }
+    // continue a checkpoint: velocities, the exact box and the neighbour-list
+    // state from the file, step numbering from its stepCount (reports append)
+    void EnableRestart() { restart = useStateVelocities = useStateBoxVectors = true; }
+    // write a restartable state file every `interval` steps (0: only at the end)
+    // and after the last step
+    void EnableCheckpoint(const std::string& filename, int interval) {
+        checkpointFilename = filename;
+        checkpointInterval = interval;
+    }
+
     // Settings of this engine with no counterpart in the reference.
     int device = 0;                 // HIP device
     double skin = 0.1;              // Verlet buffer (nm); the pair set within the cutoff is unchanged by it
     bool echo_steps = true;         // print "Simulation Step #k is completed" as the reference does
+    bool async_reports = true;      // report steps without waiting; files written by a writer thread
+    bool useStateVelocities = false, useStateBoxVectors = false, restart = false;
+    std::string checkpointFilename;
+    int checkpointInterval = 0;
 
    protected:
     virtual void initializeParameters();

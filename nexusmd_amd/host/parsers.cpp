@@ -7,8 +7,10 @@
 // (:37-130); only the first NonbondedForce is read, attributes, <Particles>
 // and <Exceptions> (:133-196); positions come from the first <Positions> of
 // <State> (StateXMLParser.cpp:17-37).  Missing attributes read as 0.
+#include <cstdio>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <stdexcept>
 
 #include "nexusmd_host.h"
@@ -165,6 +167,96 @@ std::vector<Coords3D> StateXMLParser::parseStateXML(const std::string& filename)
         return {};
     }
     return pos;
+}
+
+Coords3D StateXMLParser::extractBoxSize(const std::string& filename) {
+    try {
+        const StateData s = parseState(filename);
+        if (!s.hasBoxVectors) std::cerr << "PeriodicBoxVectors not found in XML [" << filename << "]\n";
+        return Coords3D(s.boxA[0], s.boxB[1], s.boxC[2]);
+    } catch (const std::exception& e) {
+        std::cerr << "XML [" << filename << "] parsed with errors, Error description: " << e.what() << "\n";
+        return Coords3D();
+    }
+}
+
+StateData StateXMLParser::parseState(const std::string& filename) {
+    StateData s;
+    Occurrence occ;
+    std::string err;
+    auto xyz = [](const XmlAttrs& a) { return Coords3D(a.as_double("x"), a.as_double("y"), a.as_double("z")); };
+    auto on_start = [&](const std::vector<std::string>& p, const XmlAttrs& a) {
+        occ.enter(p);
+        if (p.empty() || p[0] != "State" || occ.at(0) != 1) return;
+        if (p.size() == 1) {
+            s.stepCount = (long long)a.as_double("stepCount");
+            s.time = a.as_double("time");
+            return;
+        }
+        if (occ.at(1) != 1) return;       // the first section of each name
+        const std::string& sec = p[1];
+        if (p.size() == 2) {
+            if (sec == "NexusMDBox") {
+                s.hasBoxBounds = true;
+                s.lb = Coords3D(a.as_double("lbx"), a.as_double("lby"), a.as_double("lbz"));
+                s.ub = Coords3D(a.as_double("ubx"), a.as_double("uby"), a.as_double("ubz"));
+            } else if (sec == "NexusMDNeighborList") {
+                s.hasListState = true;
+                s.rebuildDue = a.as_int("rebuildDue");
+            } else if (sec == "Velocities") {
+                s.hasVelocities = true;
+            } else if (sec == "PeriodicBoxVectors") {
+                s.hasBoxVectors = true;
+            }
+            return;
+        }
+        if (p.size() != 3) return;
+        const std::string& item = p[2];
+        if (sec == "Positions" && item == "Position") s.positions.push_back(xyz(a));
+        else if (sec == "Velocities" && item == "Velocity") s.velocities.push_back(xyz(a));
+        else if (sec == "NexusMDNeighborList" && item == "BuildPosition") s.buildPositions.push_back(xyz(a));
+        else if (sec == "PeriodicBoxVectors" && item == "A") s.boxA = xyz(a);
+        else if (sec == "PeriodicBoxVectors" && item == "B") s.boxB = xyz(a);
+        else if (sec == "PeriodicBoxVectors" && item == "C") s.boxC = xyz(a);
+    };
+    if (!xml_scan_file(filename, on_start, &err)) throw std::runtime_error("XML [" + filename + "]: " + err);
+    return s;
+}
+
+void StateXMLParser::writeState(const std::string& filename, const StateData& s) {
+    FILE* f = std::fopen(filename.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot write " + filename);
+    std::unique_ptr<FILE, int (*)(FILE*)> hold(f, std::fclose);
+    auto row = [&](const char* tag, const Coords3D& v) {
+        std::fprintf(f, "\t\t<%s x=\"%.17g\" y=\"%.17g\" z=\"%.17g\"/>\n", tag, v[0], v[1], v[2]);
+    };
+    std::fprintf(f, "<?xml version=\"1.0\" ?>\n<State stepCount=\"%lld\" time=\"%.17g\" type=\"State\" version=\"1\">\n",
+                 s.stepCount, s.time);
+    if (s.hasBoxVectors) {
+        std::fprintf(f, "\t<PeriodicBoxVectors>\n");
+        row("A", s.boxA);
+        row("B", s.boxB);
+        row("C", s.boxC);
+        std::fprintf(f, "\t</PeriodicBoxVectors>\n");
+    }
+    if (s.hasBoxBounds)
+        std::fprintf(f, "\t<NexusMDBox lbx=\"%.17g\" lby=\"%.17g\" lbz=\"%.17g\" ubx=\"%.17g\" uby=\"%.17g\" ubz=\"%.17g\"/>\n",
+                     s.lb[0], s.lb[1], s.lb[2], s.ub[0], s.ub[1], s.ub[2]);
+    std::fprintf(f, "\t<Positions>\n");
+    for (const auto& v : s.positions) row("Position", v);
+    std::fprintf(f, "\t</Positions>\n");
+    if (s.hasVelocities) {
+        std::fprintf(f, "\t<Velocities>\n");
+        for (const auto& v : s.velocities) row("Velocity", v);
+        std::fprintf(f, "\t</Velocities>\n");
+    }
+    if (s.hasListState) {
+        std::fprintf(f, "\t<NexusMDNeighborList rebuildDue=\"%d\">\n", s.rebuildDue);
+        for (const auto& v : s.buildPositions) row("BuildPosition", v);
+        std::fprintf(f, "\t</NexusMDNeighborList>\n");
+    }
+    std::fprintf(f, "</State>\n");
+    if (std::ferror(f)) throw std::runtime_error("write error on " + filename);
 }
 
 }  // namespace nexusmd

@@ -4,6 +4,7 @@
 // so that scripts written for the reference (test/NexusMDTest.py) run
 // unchanged.  C++ exceptions surface as Python RuntimeError / ValueError.
 #include <pybind11/pybind11.h>
+#include <pybind11/numpy.h>
 #include <pybind11/stl.h>
 
 #include "nexusmd_host.h"
@@ -23,9 +24,60 @@ PYBIND11_MODULE(NexusMD, m) {
         .def("Simulate", &TaskDispatcher::Simulate, py::arg("systemFilename"), py::arg("stateFilename"),
              py::arg("inputFilename"), py::arg("outputFilename"), py::arg("StepSize"), py::arg("TotalSteps"),
              py::arg("interval"), py::call_guard<py::gil_scoped_release>())
+        .def("EnableStateVelocities", &TaskDispatcher::EnableStateVelocities)
+        .def("EnableStateBoxVectors", &TaskDispatcher::EnableStateBoxVectors)
+        .def("EnableRestart", &TaskDispatcher::EnableRestart)
+        .def("EnableCheckpoint", &TaskDispatcher::EnableCheckpoint, py::arg("filename"), py::arg("interval") = 0)
         .def_readwrite("device", &TaskDispatcher::device)
         .def_readwrite("skin", &TaskDispatcher::skin)
-        .def_readwrite("echo_steps", &TaskDispatcher::echo_steps);
+        .def_readwrite("echo_steps", &TaskDispatcher::echo_steps)
+        .def_readwrite("async_reports", &TaskDispatcher::async_reports);
+
+    // Coords3D with the reference's operations (SWIG exports it, NexusMD.i:18)
+    py::class_<Coords3D>(m, "Coords3D")
+        .def(py::init<>())
+        .def(py::init<double, double, double>(), py::arg("x"), py::arg("y"), py::arg("z"))
+        // from any sequence of three numbers (a list, tuple or numpy row), so that
+        // lists of positions pass where the C++ side takes std::vector<Coords3D>
+        .def(py::init([](const py::sequence& s) {
+            if (py::len(s) != 3) throw py::value_error("Coords3D needs three components");
+            return Coords3D(s[0].cast<double>(), s[1].cast<double>(), s[2].cast<double>());
+        }))
+        .def("__getitem__",
+             [](const Coords3D& c, int i) {
+                 if (i < 0) i += 3;
+                 if (i < 0 || i > 2) throw py::index_error("Coords3D index out of range");
+                 return c[i];
+             })
+        .def("__setitem__",
+             [](Coords3D& c, int i, double v) {
+                 if (i < 0) i += 3;
+                 if (i < 0 || i > 2) throw py::index_error("Coords3D index out of range");
+                 c[i] = v;
+             })
+        .def("__len__", [](const Coords3D&) { return 3; })
+        .def("__eq__", [](const Coords3D& a, const Coords3D& b) { return a == b; })
+        .def("__ne__", [](const Coords3D& a, const Coords3D& b) { return a != b; })
+        .def("__pos__", [](const Coords3D& a) { return +a; })
+        .def("__neg__", [](const Coords3D& a) { return -a; })
+        .def("__add__", [](const Coords3D& a, const Coords3D& b) { return a + b; })
+        .def("__sub__", [](const Coords3D& a, const Coords3D& b) { return a - b; })
+        .def("__iadd__", [](Coords3D& a, const Coords3D& b) { return a += b; })
+        .def("__isub__", [](Coords3D& a, const Coords3D& b) { return a -= b; })
+        .def("__mul__", [](const Coords3D& a, double s) { return a * s; })
+        .def("__imul__", [](Coords3D& a, double s) { return a *= s; })
+        .def("__truediv__", [](const Coords3D& a, double s) { return a / s; })
+        .def("__itruediv__", [](Coords3D& a, double s) { return a /= s; })
+        .def("dot", &Coords3D::dot)
+        .def("cross", &Coords3D::cross)
+        .def("length", &Coords3D::length)
+        .def("normalize", &Coords3D::normalize)
+        .def("__repr__", [](const Coords3D& c) {
+            return "Coords3D(" + std::to_string(c[0]) + ", " + std::to_string(c[1]) + ", " + std::to_string(c[2]) + ")";
+        });
+    py::implicitly_convertible<py::list, Coords3D>();
+    py::implicitly_convertible<py::tuple, Coords3D>();
+    py::implicitly_convertible<py::array, Coords3D>();
 
     py::class_<BondParams>(m, "BondParams")
         .def_readonly("p1", &BondParams::p1).def_readonly("p2", &BondParams::p2)
@@ -65,7 +117,28 @@ PYBIND11_MODULE(NexusMD, m) {
         .def_static("HBondParser", &SystemXMLParser::HBondParser)
         .def_static("HAngleParser", &SystemXMLParser::HAngleParser)
         .def_static("NonBondedParser", &SystemXMLParser::NonBondedParser);
-    py::class_<StateXMLParser>(m, "StateXMLParser").def_static("parseStateXML", &StateXMLParser::parseStateXML);
+    py::class_<StateData>(m, "StateData")
+        .def(py::init<>())
+        .def_readwrite("positions", &StateData::positions)
+        .def_readwrite("velocities", &StateData::velocities)
+        .def_readwrite("buildPositions", &StateData::buildPositions)
+        .def_readwrite("hasVelocities", &StateData::hasVelocities)
+        .def_readwrite("hasBoxVectors", &StateData::hasBoxVectors)
+        .def_readwrite("hasBoxBounds", &StateData::hasBoxBounds)
+        .def_readwrite("hasListState", &StateData::hasListState)
+        .def_readwrite("boxA", &StateData::boxA)
+        .def_readwrite("boxB", &StateData::boxB)
+        .def_readwrite("boxC", &StateData::boxC)
+        .def_readwrite("lb", &StateData::lb)
+        .def_readwrite("ub", &StateData::ub)
+        .def_readwrite("rebuildDue", &StateData::rebuildDue)
+        .def_readwrite("stepCount", &StateData::stepCount)
+        .def_readwrite("time", &StateData::time);
+    py::class_<StateXMLParser>(m, "StateXMLParser")
+        .def_static("parseStateXML", &StateXMLParser::parseStateXML)
+        .def_static("extractBoxSize", &StateXMLParser::extractBoxSize)
+        .def_static("parseState", &StateXMLParser::parseState)
+        .def_static("writeState", &StateXMLParser::writeState);
 
     py::class_<Reporter>(m, "Reporter")
         .def(py::init<>())

@@ -12,8 +12,12 @@
 //                      "MODEL        <n>", one line per ATOM of the input
 //                      template with coordinates x10 (nm -> Angstrom), TER,
 //                      ENDMDL
+#include <charconv>
 #include <chrono>
+#include <cmath>
+#include <cstdio>
 #include <ctime>
+#include <future>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -51,6 +55,56 @@ std::string today() {
 
 }  // namespace
 
+namespace {
+
+// Appends v in the stream's fixed notation with `prec` decimals, right-aligned
+// in `width` characters: std::to_chars in fixed notation with a precision is
+// specified to give printf's "%.*f", which is what the stream writes in the
+// "C" locale.  Non-finite values go through printf itself.
+void append_fixed(std::string& buf, double v, int prec, int width) {
+    char tmp[400];
+    int len = 0;
+    if (std::isfinite(v)) {
+        const auto r = std::to_chars(tmp, tmp + sizeof(tmp), v, std::chars_format::fixed, prec);
+        len = r.ec == std::errc() ? static_cast<int>(r.ptr - tmp) : -1;
+    }
+    if (len < 0 || !std::isfinite(v)) len = std::snprintf(tmp, sizeof(tmp), "%.*f", prec, v);
+    if (len < width) buf.append(static_cast<size_t>(width - len), ' ');
+    buf.append(tmp, static_cast<size_t>(len));
+}
+
+// The PVF rows of atoms [a0, a1): "<id>\t(x, y, z)\t(vx, vy, vz)\t(fx, fy, fz)\n",
+// fixed notation, 10 decimals, width 10.
+std::string pvf_rows(const std::vector<Coords3D>& P, const std::vector<Coords3D>& V, const std::vector<Coords3D>& F,
+                     size_t a0, size_t a1) {
+    std::string buf;
+    buf.reserve((a1 - a0) * 150);
+    auto triple = [&](const Coords3D& c) {
+        buf += '(';
+        append_fixed(buf, c[0], 10, 10);
+        buf += ", ";
+        append_fixed(buf, c[1], 10, 10);
+        buf += ", ";
+        append_fixed(buf, c[2], 10, 10);
+        buf += ')';
+    };
+    char id[32];
+    for (size_t i = a0; i < a1; ++i) {
+        const auto r = std::to_chars(id, id + sizeof(id), i + 1);
+        buf.append(id, static_cast<size_t>(r.ptr - id));
+        buf += '\t';
+        triple(P[i]);
+        buf += '\t';
+        triple(V[i]);
+        buf += '\t';
+        triple(F[i]);
+        buf += '\n';
+    }
+    return buf;
+}
+
+}  // namespace
+
 void Reporter::TestPVFReport(const std::string& filename, const std::vector<Coords3D>& positions,
                              const std::vector<Coords3D>& velocities, const std::vector<Coords3D>& forces, int step) {
     const std::string name = "PVFReport" + filename;
@@ -59,20 +113,23 @@ void Reporter::TestPVFReport(const std::string& filename, const std::vector<Coor
         std::cerr << "Unable to open the file: " << name << std::endl;
         return;
     }
-    out << "Step" << step << "\n";
-    out << "ID\tPosition(nm)\t\t\tVelocity(nm / ps)\t\tForce(kJ / mol * nm)\n";
-    out << std::fixed << std::setprecision(10);
-    auto triple = [&](const Coords3D& v) {
-        out << "(" << std::setw(10) << v[0] << ", " << std::setw(10) << v[1] << ", " << std::setw(10) << v[2] << ")";
-    };
-    for (size_t i = 0; i < positions.size(); ++i) {
-        out << i + 1 << "\t";
-        triple(positions[i]);
-        out << "\t";
-        triple(velocities[i]);
-        out << "\t";
-        triple(forces[i]);
-        out << "\n";
+    if (velocities.size() < positions.size() || forces.size() < positions.size())
+        throw std::invalid_argument("TestPVFReport: fewer velocities or forces than positions");
+    std::string head = "Step" + std::to_string(step) + "\n";
+    head += "ID\tPosition(nm)\t\t\tVelocity(nm / ps)\t\tForce(kJ / mol * nm)\n";
+    out.write(head.data(), static_cast<std::streamsize>(head.size()));
+    // rows formatted in parallel blocks for large systems, written in order
+    const size_t n = positions.size();
+    const size_t nblk = n >= 16384 ? std::min<size_t>(8, n / 4096) : 1;
+    std::vector<std::future<std::string>> parts;
+    for (size_t b = 1; b < nblk; ++b)
+        parts.push_back(std::async(std::launch::async, pvf_rows, std::cref(positions), std::cref(velocities),
+                                   std::cref(forces), n * b / nblk, n * (b + 1) / nblk));
+    const std::string first = pvf_rows(positions, velocities, forces, 0, n / nblk);
+    out.write(first.data(), static_cast<std::streamsize>(first.size()));
+    for (auto& f : parts) {
+        const std::string rows = f.get();
+        out.write(rows.data(), static_cast<std::streamsize>(rows.size()));
     }
     out << "\n";
 }
@@ -103,7 +160,15 @@ PDBAtom Reporter::parsePDBLine(const std::string& line) {
 }
 
 std::string Reporter::formatPDBLine(const PDBAtom& a) {
-    std::ostringstream ss;
+    // the reference's manipulator chain as one C format: left 6 / right 5 / left 4 /
+    // right 3 / char / right 4, then fixed 8.3 x3 and 6.2 x2, left 2
+    char line[256];
+    const int len = std::snprintf(line, sizeof(line), "%-6s%5d %-4s %3s %c%4d    %8.3f%8.3f%8.3f%6.2f%6.2f          %-2s",
+                                  a.recordType.c_str(), a.atomNum, a.atomName.c_str(), a.resName.c_str(), a.chainID,
+                                  a.resSeq, a.coords[0], a.coords[1], a.coords[2], a.occupancy, a.tempFactor,
+                                  a.element.c_str());
+    if (len > 0 && static_cast<size_t>(len) < sizeof(line)) return std::string(line, static_cast<size_t>(len));
+    std::ostringstream ss;     // very long fields: the stream chain itself
     ss << std::left << std::setw(6) << a.recordType << std::right << std::setw(5) << a.atomNum << " " << std::left
        << std::setw(4) << a.atomName << std::right << std::setw(1) << " " << std::setw(3) << a.resName << " "
        << std::setw(1) << a.chainID << std::setw(4) << std::right << a.resSeq << std::setw(1) << " "
@@ -143,21 +208,35 @@ void Reporter::pdbOutputGeneratorPart2(const std::string& outputFilename, const 
         return;
     }
     std::cout << ModelNum << "   ";
-    out << "MODEL        " << ModelNum << std::endl;
+    // the model is formatted into one buffer and written at once (the same bytes
+    // as a flush after every line)
+    std::string buf = "MODEL        " + std::to_string(ModelNum) + "\n";
+    buf.reserve(buf.size() + (tmpl.size() + 2) * 81);
     for (size_t i = 0; i < tmpl.size(); ++i) {
         PDBAtom a = tmpl[i];
         const size_t k = static_cast<size_t>(a.atomNum - 1);
         if (a.atomNum < 1 || k >= positions.size())
             throw std::runtime_error("PDB atom number " + std::to_string(a.atomNum) + " has no position");
         a.coords = Coords3D{positions[k][0] * 10.0, positions[k][1] * 10.0, positions[k][2] * 10.0};
-        out << formatPDBLine(a) << std::endl;
+        buf += formatPDBLine(a);
+        buf += "\n";
         if (i + 1 == tmpl.size()) {
-            out << std::left << std::setw(6) << "TER" << std::right << std::setw(5) << a.atomNum + 1 << " "
-                << std::left << std::setw(4) << "" << std::right << std::setw(1) << " " << std::setw(3) << a.resName
-                << " " << std::setw(1) << a.chainID << std::setw(4) << std::right << a.resSeq << std::endl;
+            char ter[128];
+            const int len = std::snprintf(ter, sizeof(ter), "%-6s%5d %-4s %3s %c%4d\n", "TER", a.atomNum + 1, "",
+                                          a.resName.c_str(), a.chainID, a.resSeq);
+            if (len > 0 && static_cast<size_t>(len) < sizeof(ter)) {
+                buf.append(ter, static_cast<size_t>(len));
+            } else {
+                std::ostringstream ss;
+                ss << std::left << std::setw(6) << "TER" << std::right << std::setw(5) << a.atomNum + 1 << " "
+                   << std::left << std::setw(4) << "" << std::right << std::setw(1) << " " << std::setw(3)
+                   << a.resName << " " << std::setw(1) << a.chainID << std::setw(4) << std::right << a.resSeq << "\n";
+                buf += ss.str();
+            }
         }
     }
-    out << "ENDMDL" << std::endl;
+    buf += "ENDMDL\n";
+    out.write(buf.data(), static_cast<std::streamsize>(buf.size()));
 }
 
 }  // namespace nexusmd

@@ -64,7 +64,10 @@ def write_system_xml(sys_, path: str, use_switch: int = 0, switch_distance: floa
         f.write("\n".join(out))
 
 
-def write_state_xml(sys_, path: str, pos=None) -> None:
+def write_state_xml(sys_, path: str, pos=None, velocities=False) -> None:
+    """<State> with <PeriodicBoxVectors> and <Positions>; with velocities=True
+    also <Velocities> (OpenMM's layout), which the dispatcher reads only when
+    EnableStateVelocities is on."""
     pos = np.asarray(sys_.pos if pos is None else pos, np.float64)
     L = sys_.L
     out = ['<?xml version="1.0" ?>',
@@ -76,7 +79,13 @@ def write_state_xml(sys_, path: str, pos=None) -> None:
            "\t</PeriodicBoxVectors>",
            "\t<Positions>"]
     out += ['\t\t<Position x="%s" y="%s" z="%s"/>' % (_f(p[0]), _f(p[1]), _f(p[2])) for p in pos]
-    out += ["\t</Positions>", "</State>", ""]
+    out += ["\t</Positions>"]
+    if velocities:
+        out += ["\t<Velocities>"]
+        out += ['\t\t<Velocity x="%s" y="%s" z="%s"/>' % (_f(v[0]), _f(v[1]), _f(v[2]))
+                for v in np.asarray(sys_.vel, np.float64)]
+        out += ["\t</Velocities>"]
+    out += ["</State>", ""]
     with open(path, "w") as f:
         f.write("\n".join(out))
 

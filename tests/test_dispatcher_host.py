@@ -245,3 +245,102 @@ def test_simulate_rejects_bad_arguments_before_touching_a_device(nx, c0_files):
         d.Simulate(sysxml, stxml, pdb, "out.pdb", 0.0, 10, 1)
     with pytest.raises(ValueError):
         d.Simulate(sysxml, stxml, pdb, "out.pdb", 0.001, 10, 0)
+
+
+def test_coords3d_has_the_reference_operations(nx):
+    """Coords3D as the SWIG wrapper exports it (wrapper/NexusMD.i:18;
+    core/common/headers/Coords3D.h:15-151): components by index, + - * /,
+    dot, cross, length, normalize (zero stays zero)."""
+    a, b = nx.Coords3D(1.0, 2.0, 3.0), nx.Coords3D(-2.0, 0.5, 4.0)
+    assert (a[0], a[1], a[2], a[-1]) == (1.0, 2.0, 3.0, 3.0)
+    assert list(a + b) == [-1.0, 2.5, 7.0] and list(a - b) == [3.0, 1.5, -1.0]
+    assert list(-a) == [-1.0, -2.0, -3.0] and list(a * 2.0) == [2.0, 4.0, 6.0]
+    assert list(nx.Coords3D(3.0, 6.0, 9.0) / 3.0) == [1.0, 2.0, 3.0]
+    assert a.dot(b) == 1.0 * -2.0 + 2.0 * 0.5 + 3.0 * 4.0
+    assert list(a.cross(b)) == list(np.cross([1.0, 2.0, 3.0], [-2.0, 0.5, 4.0]))
+    assert a.length() == pytest.approx(np.sqrt(14.0))
+    assert list(nx.Coords3D(0.0, 3.0, 4.0).normalize()) == [0.0, 0.6, 0.8]
+    assert list(nx.Coords3D().normalize()) == [0.0, 0.0, 0.0]
+    c = nx.Coords3D(1.0, 1.0, 1.0)
+    c += a
+    c *= 2.0
+    assert c == nx.Coords3D(4.0, 6.0, 8.0) and c != a
+    c[1] = -1.0
+    assert c[1] == -1.0
+    with pytest.raises(IndexError):
+        a[3]
+
+
+def test_state_file_round_trip(nx, tmp_path):
+    """writeState then parseState gives back every double exactly (17
+    significant digits), the OpenMM sections and the restart extension."""
+    rng = np.random.default_rng(5)
+    n = 40
+    st = nx.StateData()
+    st.positions = rng.normal(size=(n, 3)).tolist()
+    st.velocities = (rng.normal(size=(n, 3)) * 1e-7).tolist()
+    st.buildPositions = (rng.normal(size=(n, 3)) + 1.0 / 3.0).tolist()
+    st.hasVelocities = st.hasBoxVectors = st.hasBoxBounds = st.hasListState = True
+    st.boxA, st.boxB, st.boxC = [2.1, 0.0, 0.0], [0.0, 3.0000000000000004, 0.0], [0.0, 0.0, np.pi]
+    st.lb, st.ub = [-0.1, 0.0, 1e-17], [2.0, 3.0000000000000004, np.pi + 1e-17]
+    st.rebuildDue = 1
+    st.stepCount = 123456789012
+    st.time = 61728.394506
+    path = str(tmp_path / "ck.xml")
+    nx.StateXMLParser.writeState(path, st)
+    r = nx.StateXMLParser.parseState(path)
+    for k in ("positions", "velocities", "buildPositions"):
+        assert np.array_equal(np.array(getattr(r, k)), np.array(getattr(st, k))), k
+    for k in ("boxA", "boxB", "boxC", "lb", "ub"):
+        assert list(getattr(r, k)) == list(getattr(st, k)), k
+    assert (r.hasVelocities, r.hasBoxVectors, r.hasBoxBounds, r.hasListState) == (True, True, True, True)
+    assert (r.rebuildDue, r.stepCount, r.time) == (1, 123456789012, 61728.394506)
+    # the reference's readers see an ordinary OpenMM state
+    assert np.array_equal(np.array(nx.StateXMLParser.parseStateXML(path)), np.array(st.positions))
+    assert list(nx.StateXMLParser.extractBoxSize(path)) == [2.1, 3.0000000000000004, np.pi]
+    root = ET.parse(path).getroot()
+    assert root.tag == "State" and root.find("Positions") is not None and root.find("Velocities") is not None
+
+
+def test_openmm_state_with_velocities(nx, tmp_path):
+    """A plain OpenMM state (no extension): positions, velocities and box vectors read,
+    no box bounds and no list state."""
+    s = S.bead_chain(n=50, L=2.5, grid=(24, 24, 24), seed=4)
+    path = str(tmp_path / "s.xml")
+    xmlio.write_state_xml(s, path, velocities=True)
+    r = nx.StateXMLParser.parseState(path)
+    assert np.array_equal(np.array(r.positions), s.pos) and np.array_equal(np.array(r.velocities), s.vel)
+    assert r.hasVelocities and r.hasBoxVectors and not r.hasBoxBounds and not r.hasListState
+    assert list(r.boxA) == [2.5, 0.0, 0.0] and r.stepCount == 0
+
+
+def test_pvf_number_format_on_extreme_values(nx, tmp_path):
+    """TestPVFReport's numbers are the stream's fixed notation, 10 decimals,
+    width 10 (Reporter.cpp:45-54): checked against printf-style "%10.10f" on
+    values across magnitudes, signed zeros, ties in the last decimal and
+    non-finite values, in a system large enough to be formatted in parallel
+    blocks."""
+    rng = np.random.default_rng(11)
+    n = 20000
+    vals = rng.normal(size=(n, 9)) * 10.0 ** rng.integers(-14, 16, size=(n, 9))
+    special = [0.0, -0.0, 0.5e-10, 1.5e-10, 2.5e-10, -2.5e-10, 1e300, -1e-300, 123456789.00000000005,
+               float("inf"), float("-inf"), float("nan"), 5e-324, 0.12345678905, 9.99999999995]
+    vals.ravel()[: len(special)] = special
+    p, v, f = vals[:, 0:3].tolist(), vals[:, 3:6].tolist(), vals[:, 6:9].tolist()
+    rep = nx.Reporter()
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        rep.TestPVFReport("x.pdb", p, v, f, 0)
+    finally:
+        os.chdir(cwd)
+    lines = open(tmp_path / "PVFReportx.pdb").read().split("\n")
+    assert lines[0] == "Step0" and lines[-1] == "" and lines[-2] == ""
+
+    def fx(x):
+        return "%10.10f" % x
+
+    for i in range(n):
+        r = vals[i]
+        want = "%d\t(%s, %s, %s)\t(%s, %s, %s)\t(%s, %s, %s)" % ((i + 1,) + tuple(fx(x) for x in r))
+        assert lines[2 + i] == want, (i, lines[2 + i], want)
