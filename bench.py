@@ -376,6 +376,56 @@ def serving_bench(n_requests, steps_per_request, alone_steps, alone_full_sample=
     }
 
 
+def reports_bench(steps_c0=400, steps_c1=60, workdir=None):
+    """TaskDispatcher::Simulate with a report on every step (interval 1, the
+    reference's own script's setting) through the NexusMD module, on the
+    1,500-bead chain and the 24,000-atom water box: ns/day with the report
+    steps enqueued without waiting and files written by a writer thread
+    (default), and with the synchronous report path."""
+    import tempfile
+
+    from nexusmd_amd import build as B
+    from nexusmd_amd import systems as S
+    from nexusmd_amd import xmlio
+    sys.path.insert(0, os.path.dirname(B.module_path()))
+    import NexusMD
+    out = {}
+    for name, mk, steps, dt in (("bead_chain_1500", S.bead_chain, steps_c0, 0.001),
+                                ("water_24000", S.water_box, steps_c1, 0.0005)):
+        s = mk()
+        d = tempfile.mkdtemp(prefix="nx_reports_", dir=workdir)
+        xmlio.write_system_xml(s, os.path.join(d, "system.xml"))
+        xmlio.write_state_xml(s, os.path.join(d, "state.xml"))
+        xmlio.write_pdb(s, os.path.join(d, "input.pdb"))
+        row = {"atoms": s.n, "steps": steps, "interval": 1, "dt_fs": dt * 1e3}
+        for mode in (True, False):
+            disp = NexusMD.TaskDispatcher.CreateDispatcher("GPU_MIXED")
+            disp.echo_steps = False
+            disp.async_reports = mode
+            for en in (disp.EnableHarmonicBondForce, disp.EnableHarmonicAngleForce,
+                       disp.EnablePeriodicTorsionForce, disp.EnableNonbondedForce):
+                en()
+            cwd = os.getcwd()
+            os.chdir(d)
+            try:
+                disp.Simulate("system.xml", "state.xml", "input.pdb", "out.pdb", dt, 5, 1)     # warm-up
+                t0 = time.perf_counter()
+                disp.Simulate("system.xml", "state.xml", "input.pdb", "out.pdb", dt, steps, 1)
+                wall = time.perf_counter() - t0
+            finally:
+                os.chdir(cwd)
+            key = "async" if mode else "sync"
+            row[key + "_ns_per_day"] = ns_per_day(steps, dt, wall)
+            row[key + "_ms_per_step"] = 1e3 * wall / steps
+        row["report_bytes_per_step"] = sum(os.path.getsize(os.path.join(d, f))
+                                           for f in ("out.pdb", "PVFReportout.pdb", "TotalEnergyout.pdb")) // steps
+        import shutil
+        shutil.rmtree(d, ignore_errors=True)
+        log("reports %s: %s" % (name, row))
+        out[name] = row
+    return out
+
+
 PROFILE = os.path.join(ROOT, "profiles", "kernels_solvated_chain_100000.md")
 
 
@@ -439,6 +489,8 @@ def main():
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving measurement")
     ap.add_argument("--serving-only", action="store_true", help="run only the batched-serving measurement")
+    ap.add_argument("--reports-only", action="store_true",
+                    help="run only Simulate with a report on every step (c0 and c1 sizes), async and sync reports")
     ap.add_argument("--serving-requests", type=int, default=256)
     ap.add_argument("--serving-steps", type=int, default=5000)
     ap.add_argument("--alone-steps", type=int, default=200)
@@ -462,6 +514,9 @@ def main():
     if args.serving_only:
         sv = serving_bench(args.serving_requests, args.serving_steps, args.alone_steps)
         print(json.dumps({"serving": sv}), flush=True)
+        return
+    if args.reports_only:
+        print(json.dumps({"reports": reports_bench()}), flush=True)
         return
 
     t = time.perf_counter()
