@@ -95,7 +95,7 @@ typedef struct nx_ctx nx_ctx;
 nx_status nx_create(int device, int precision, nx_ctx** out);
 nx_status nx_destroy(nx_ctx* ctx);
 /* Text of the context's last error; with NULL, the calling thread's last error
- * of a context-free call (nx_host_alloc, nx_host_free). */
+ * of a context-free call (nx_host_alloc, nx_host_free, nx_fft3d_r2c_noalloc). */
 const char* nx_last_error(const nx_ctx* ctx);
 const char* nx_status_string(nx_status s);
 
@@ -223,6 +223,21 @@ nx_status nx_get_stats(nx_ctx* ctx, nx_stats* s);
  * back to real.  precision as nx_create. */
 nx_status nx_fft3d_r2c(int device, int precision, int nx, int ny, int nz, const double* in, double* out);
 nx_status nx_fft3d_c2r(int device, int precision, int nx, int ny, int nz, const double* in, double* out);
+
+/* nx_fft3d_r2c without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code around graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * result and bits as nx_fft3d_r2c; every device buffer (grid, spectrum,
+ * twiddles, work items) lives in a 20 MiB array that each GPU holds from the
+ * library's load until the process ends.  Limit: every dimension <= 256 and
+ * at most 2^20 grid points (e.g. 96^3, 64x128x128); larger grids return
+ * NX_ERR_CAPACITY.  The array keeps the last call's contents; calls on one GPU
+ * share it and the library holds a per-device lock through each call, so
+ * calls from different threads run one after the other (anything else that
+ * wrote to the array during a call would race with it).  Runs on the device's
+ * null stream and returns after the result is in `out`.  The reason of a
+ * failure is nx_last_error(NULL) on the calling thread. */
+nx_status nx_fft3d_r2c_noalloc(int device, int precision, int nx, int ny, int nz, const double* in, double* out);
 
 /* ---------------------------------------------------------------- serving
  * Continuous batching of many independent small systems on one GPU (no

@@ -106,6 +106,7 @@ SIGNATURES = [
     ("nx_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning), D, D]),
     ("nx_fft3d_r2c", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D]),
     ("nx_fft3d_c2r", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D]),
+    ("nx_fft3d_r2c_noalloc", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D]),
     ("nx_server_create", C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
     ("nx_server_destroy", C.c_int, [C.c_void_p]),
     ("nx_server_last_error", C.c_char_p, [C.c_void_p]),

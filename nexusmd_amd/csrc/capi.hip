@@ -527,4 +527,27 @@ nx_status nx_fft3d_c2r(int device, int precision, int nx_, int ny, int nz, const
     return fft_common(device, precision, nx_, ny, nz, in, out, false);
 }
 
+nx_status nx_fft3d_r2c_noalloc(int device, int precision, int nx_, int ny, int nz, const double* in, double* out) {
+    auto set = [](nx_status s, const std::string& m) {
+        tl_error = m;
+        return s;
+    };
+    if (!in || !out) return set(NX_ERR_ARG, "nx_fft3d_r2c_noalloc: input or output is NULL");
+    if (precision != NX_PRECISION_FP64 && precision != NX_PRECISION_MIXED)
+        return set(NX_ERR_ARG, "nx_fft3d_r2c_noalloc: unknown precision");
+    auto ok = [](int K) {
+        int m = K;
+        for (int p : {2, 3, 5})
+            while (m % p == 0) m /= p;
+        return K >= 2 && m == 1;
+    };
+    if (!ok(nx_) || !ok(ny) || !ok(nz))
+        return set(NX_ERR_ARG, "nx_fft3d_r2c_noalloc: every dimension must be >= 2 with prime factors 2, 3, 5 only");
+    std::string err;
+    const nx_status s = precision == NX_PRECISION_MIXED
+                            ? nx::fft3d_r2c_noalloc<nx::PrecMixed>(device, nx_, ny, nz, in, out, err)
+                            : nx::fft3d_r2c_noalloc<nx::PrecFP64>(device, nx_, ny, nz, in, out, err);
+    return set(s, s == NX_OK ? std::string() : "nx_fft3d_r2c_noalloc: " + err);
+}
+
 }  // extern "C"

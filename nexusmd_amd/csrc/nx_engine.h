@@ -156,5 +156,9 @@ template <class P> void launch_export_pairs(Batch<P>& B, int2* out, long long ca
 // standalone FFT (verification)
 template <class P> void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool forward,
                                         hipStream_t st);
+// the forward transform of fft3d_roundtrip in the library's static device
+// scratch array (no allocation); err receives the reason of a failure
+template <class P> nx_status fft3d_r2c_noalloc(int device, int nx, int ny, int nz, const double* in, double* out,
+                                               std::string& err);
 
 }  // namespace nx

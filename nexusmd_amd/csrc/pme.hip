@@ -1100,4 +1100,176 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
 template void fft3d_roundtrip<PrecFP64>(int, int, int, const double*, double*, bool, hipStream_t);
 template void fft3d_roundtrip<PrecMixed>(int, int, int, const double*, double*, bool, hipStream_t);
 
+// ------------------------------------------------------ no-allocation FFT
+//
+// The forward transform of fft3d_roundtrip with every device buffer carved
+// out of one file-scope array, `scratch`, so that a call allocates nothing.
+// Each GPU holds its own copy of the array from the library's load until the
+// process ends.  Layout (256-byte aligned regions):
+//   [SysDev][twiddles, 3 x kNoAllocMaxK][work items, kNoAllocItems int4]
+//   [real grid, 8 B per point][half spectrum, sizeof(real2) per point]
+// The input is copied straight into the real-grid region (the mixed path
+// then converts it to fixed point in place, element by element), and the
+// FP64 spectrum is copied straight out of its region; the mixed spectrum
+// (float pairs) is copied into the caller's output and widened there in
+// place, from the last element backwards.  The host side uses the device
+// address returned by hipGetSymbolAddress, never the array's name.
+constexpr int kNoAllocMaxK = 256;
+constexpr long long kNoAllocMaxPoints = 1LL << 20;
+constexpr int kNoAllocItems = 32768;
+constexpr size_t kNoAllocBytes = 20u << 20;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_noalloc_mu[kNoAllocMaxDevices];     // one call at a time per device
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};    // device address of each device's array
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Work items of the forward passes, in fft3d_roundtrip's order.  Plane passes:
+// x planes, then y rows of the x pass over every kz.  Line passes: groups of
+// T z lines, then (x, kz block) and (y, kz block) items.
+__global__ void noalloc_items_kernel(int4* it, int nx, int ny, int nzh, int T, int plane, int n0, int n1, int n2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n0 + n1 + n2) return;
+    if (plane) {
+        if (i < n0) it[i] = make_int4(0, i, 0, 0);
+        else it[i] = make_int4(0, i - n0, 0, nzh);
+        return;
+    }
+    const int nkb = (nzh + T - 1) / T;
+    if (i < n0) {
+        const int l = i * T;
+        it[i] = make_int4(0, l, min(T, nx * ny - l), 0);
+    } else if (i < n0 + n1) {
+        const int j = i - n0;
+        it[i] = make_int4(0, j / nkb, (j % nkb) * T, 0);
+    } else {
+        const int j = i - n0 - n1;
+        it[i] = make_int4(0, j / nkb, (j % nkb) * T, 0);
+    }
+}
+
+// In place: the doubles of the input become 2^32 fixed point (as the host
+// conversion of fft3d_roundtrip: round to nearest, ties to even).
+__global__ void noalloc_to_fixed_kernel(void* g, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = reinterpret_cast<const double*>(g)[i];
+    reinterpret_cast<long long*>(g)[i] = __double2ll_rn(v * kForceScale);
+}
+
+}  // namespace
+
+template <class P>
+nx_status fft3d_r2c_noalloc(int device, int nx, int ny, int nz, const double* in, double* out, std::string& err) {
+    using R2 = typename P::real2;
+    using real = typename P::real;
+    if (device < 0 || device >= kNoAllocMaxDevices) { err = "device index out of range"; return NX_ERR_ARG; }
+    const int nzh = nz / 2 + 1;
+    const long long G = (long long)nx * ny * nz, Gh = (long long)nx * ny * nzh;
+    if (nx > kNoAllocMaxK || ny > kNoAllocMaxK || nz > kNoAllocMaxK || G > kNoAllocMaxPoints) {
+        err = "grid " + std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) +
+              " exceeds the no-allocation maximum (each dimension <= " + std::to_string(kNoAllocMaxK) + ", at most " +
+              std::to_string(kNoAllocMaxPoints) + " points)";
+        return NX_ERR_CAPACITY;
+    }
+    const int maxK = std::max(nx, std::max(ny, nz));
+    const int T = std::max(1, std::min(8, (int)((61440 / (int)sizeof(R2) - maxK) / (2 * maxK))));
+    const bool plane = fft_plane_enabled() && plane_lds_bytes<R2>(ny, nz) <= kMaxBlockLds &&
+                       xpass_lds_bytes<R2>(nx, nzh) <= kMaxBlockLds;
+    const int nkb = (nzh + T - 1) / T;
+    const int n0 = plane ? nx : (nx * ny + T - 1) / T;
+    const int n1 = plane ? ny : nx * nkb;
+    const int n2 = plane ? 0 : ny * nkb;
+    const size_t o_tw = align256(sizeof(SysDev));
+    const size_t o_it = o_tw + align256(3 * kNoAllocMaxK * sizeof(R2));
+    const size_t o_rg = o_it + align256((size_t)kNoAllocItems * sizeof(int4));
+    const size_t o_cg = o_rg + align256((size_t)G * 8);
+    const size_t need = o_cg + (size_t)Gh * sizeof(R2);
+    if (n0 + n1 + n2 > kNoAllocItems || need > kNoAllocBytes) {
+        err = "grid " + std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) + " needs " +
+              std::to_string(need) + " bytes of scratch; the no-allocation array holds " + std::to_string(kNoAllocBytes);
+        return NX_ERR_CAPACITY;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu[device]);
+    auto fail = [&](hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return NX_ERR_HIP;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(e, "hipSetDevice");
+    if (!g_noalloc_ptr[device]) {
+        // once per device, with that device current: the address of its copy
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(e, "hipGetSymbolAddress(scratch)");
+        g_noalloc_ptr[device] = p;
+    }
+    unsigned char* base = static_cast<unsigned char*>(g_noalloc_ptr[device]);
+    SysDev* dd = reinterpret_cast<SysDev*>(base);
+    R2* dtw = reinterpret_cast<R2*>(base + o_tw);
+    int4* items = reinterpret_cast<int4*>(base + o_it);
+    typename P::facc* rg = reinterpret_cast<typename P::facc*>(base + o_rg);
+    R2* cg = reinterpret_cast<R2*>(base + o_cg);
+    const hipStream_t st = nullptr;
+    // the per-call tables, from host values on the stack
+    SysDev d{};
+    d.grid[0] = nx; d.grid[1] = ny; d.grid[2] = nz;
+    d.tw_off[0] = 0; d.tw_off[1] = nx; d.tw_off[2] = nx + ny;
+    R2 tw[3 * kNoAllocMaxK];
+    const int K[3] = {nx, ny, nz};
+    for (int a = 0, o = 0; a < 3; o += K[a], ++a)
+        for (int t = 0; t < K[a]; ++t) {
+            const double ang = -2.0 * kPi * (double)t / (double)K[a];
+            tw[o + t].x = (real)cos(ang);
+            tw[o + t].y = (real)sin(ang);
+        }
+    if ((e = hipMemcpyAsync(dd, &d, sizeof(SysDev), hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e, "copy");
+    if ((e = hipMemcpyAsync(dtw, tw, (size_t)(nx + ny + nz) * sizeof(R2), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return fail(e, "copy");
+    noalloc_items_kernel<<<(n0 + n1 + n2 + 255) / 256, 256, 0, st>>>(items, nx, ny, nzh, T, plane ? 1 : 0, n0, n1, n2);
+    if ((e = hipMemcpyAsync(rg, in, (size_t)G * sizeof(double), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return fail(e, "copy");
+    if constexpr (P::mixed) noalloc_to_fixed_kernel<<<(unsigned)((G + 255) / 256), 256, 0, st>>>(rg, G);
+    if (plane) {
+        const size_t pl = plane_lds_bytes<R2>(ny, nz), xl = xpass_lds_bytes<R2>(nx, nzh);
+        if ((e = hipFuncSetAttribute((const void*)fft_plane_fwd<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)pl)) != hipSuccess ||
+            (e = hipFuncSetAttribute((const void*)fft_x_block<P, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)xl)) != hipSuccess)
+            return fail(e, "hipFuncSetAttribute");
+        fft_plane_fwd<P><<<(unsigned)n0, kPlaneThreads, pl, st>>>(dd, items, rg, cg, dtw);
+        fft_x_block<P, 0><<<(unsigned)n1, kXPassThreads, xl, st>>>(dd, items + n0, cg, nullptr, dtw, nullptr);
+    } else {
+        const size_t sm = (size_t)(2 * T + 1) * maxK * sizeof(R2);
+        fft_z_fwd<P><<<(unsigned)n0, 256, sm, st>>>(dd, items, rg, cg, dtw);
+        fft_y<P><<<(unsigned)n1, 256, sm, st>>>(dd, items + n0, cg, dtw, T, -1);
+        fft_x_plain<P><<<(unsigned)n2, 256, sm, st>>>(dd, items + n0 + n1, cg, dtw, T, -1);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "kernel launch");
+    if ((e = hipMemcpyAsync(out, cg, (size_t)Gh * sizeof(R2), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return fail(e, "copy");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+    if constexpr (P::mixed) {
+        // float pairs at the front of `out` widened to doubles in place, last first:
+        // element i is read from bytes [8i, 8i + 8) before bytes [16i, 16i + 16) are
+        // written, and no element below i is read from at or above byte 8i
+        unsigned char* ob = reinterpret_cast<unsigned char*>(out);
+        for (long long i = Gh - 1; i >= 0; --i) {
+            float v[2];
+            std::memcpy(v, ob + 8 * i, sizeof(v));
+            const double w[2] = {(double)v[0], (double)v[1]};
+            std::memcpy(ob + 16 * i, w, sizeof(w));
+        }
+    }
+    return NX_OK;
+}
+
+template nx_status fft3d_r2c_noalloc<PrecFP64>(int, int, int, int, const double*, double*, std::string&);
+template nx_status fft3d_r2c_noalloc<PrecMixed>(int, int, int, int, const double*, double*, std::string&);
+
 }  // namespace nx
