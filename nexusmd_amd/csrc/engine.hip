@@ -210,6 +210,10 @@ class Engine : public EngineBase {
         // PME runs on a second stream beside the pair kernel; NEXUSMD_OVERLAP=0 serialises them
         const char* o = std::getenv("NEXUSMD_OVERLAP");
         if (o && std::string(o) == "0") overlap_ = false;
+        // NEXUSMD_PAIR_IMAGE=0: the mixed pair kernel takes every pair's nearest
+        // image itself instead of the list's per-entry image
+        const char* pi = std::getenv("NEXUSMD_PAIR_IMAGE");
+        if (pi && std::string(pi) == "0") pair_image_ = false;
         HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
@@ -284,6 +288,7 @@ class Engine : public EngineBase {
             }
             d.n_atoms = n;
             d.atom_off = (int)atom_off;
+            d.pair_image = pair_image_ ? 1 : 0;
             // columns of ~8/rho^(1/3) nm so that a cluster of 8 is roughly cubic
             const double V = d.L[0] * d.L[1] * d.L[2];
             const double rho = std::max(1e-3, n / V);
@@ -308,7 +313,7 @@ class Engine : public EngineBase {
             double ext = (std::min(d.L[0], w + 2 * r + 2 * w)) * (std::min(d.L[1], w + 2 * r + 2 * w)) *
                          std::min(d.L[2], hz + 2 * r + 0.5);
             double per_super = 1.5 * (rho / 8.0) * ext * h.list_scale;
-            per_super = std::min(per_super, 1024.0);
+            per_super = std::min(per_super, (double)kSuperListCap);
             per_super = std::max(per_super, 16.0);
             d.list_off = (int)list_off;
             d.list_cap = (int)std::min<double>(2.0e9, per_super * d.super_cap + 64);
@@ -383,7 +388,9 @@ class Engine : public EngineBase {
             emask_off += d.emask_cap;
             excl_off_total += (long long)h.excl_idx.size();
         }
-        if (slot_off > (1 << 27) || list_off > 2000000000LL) throw HipError(NX_ERR_ARG, "batch too large");
+        // cluster ids fill 24 bits of a list word, exclusion-mask ids 26
+        if (slot_off > (1 << 27) || list_off > 2000000000LL || emask_off >= (long long)kNoMask)
+            throw HipError(NX_ERR_ARG, "batch too large");
         B.n_atoms = (int)atom_off;
         B.n_slots_cap = (int)slot_off;
         B.n_cols = (int)col_off;
@@ -807,6 +814,8 @@ class Engine : public EngineBase {
     }
 
     // The bonded and exclusion terms and PME (the work beside the pair kernel).
+    // (Starting the terms earlier, beside the list work, measured slower: 0.263
+    // against 0.240 ms per 100k-atom mixed step.)
     void side_work(int mask, bool energy) {
         if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
         timed("atom_forces", [&] { launch_atom_forces<P>(B, mask, energy); });
@@ -1479,6 +1488,7 @@ class Engine : public EngineBase {
     hipStream_t side_ = nullptr;            // PME beside the pair kernel
     hipEvent_t fork_ = nullptr, join_ = nullptr;
     bool overlap_ = true;
+    bool pair_image_ = true;
     bool pack_needed_ = false;              // slot positions/forces stale (see settle)
     double graph_dt_seg_ = 0.0;
     int graph_mask_seg_ = -1;

@@ -31,8 +31,9 @@ template <class P>
 __global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
                             const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
                             const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
-                            const double* __restrict__ Q, typename P::real4* pos, typename P::facc* FX,
-                            typename P::facc* FY, typename P::facc* FZ, int n_slots_total) {
+                            const double* __restrict__ XB, const double* __restrict__ YB,
+                            const double* __restrict__ ZB, const double* __restrict__ Q, typename P::real4* pos,
+                            typename P::facc* FX, typename P::facc* FY, typename P::facc* FZ, int n_slots_total) {
     using real = typename P::real;
     const int sl = blockIdx.x * blockDim.x + threadIdx.x;
     if (sl >= n_slots_total) return;
@@ -40,10 +41,11 @@ __global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __res
     const int a = (sl - sd[s].slot_off < sc[s].n_slots) ? atom_of_slot[sl] : -1;
     typename P::real4 p;
     if (a >= 0) {
-        p = slot_position<P>(X[a], Y[a], Z[a], Q[a]);
+        double c[3] = {X[a], Y[a], Z[a]};
+        if constexpr (P::mixed) continuous_position(sd[s], X[a], Y[a], Z[a], XB[a], YB[a], ZB[a], c);
+        p = slot_position<P>(c[0], c[1], c[2], Q[a]);
     } else {
-        const real nan = __builtin_nan("");
-        p.x = p.y = p.z = nan;
+        p.x = p.y = p.z = (real)kPadPos;
         p.w = 0;
     }
     pos[sl] = p;
@@ -141,10 +143,13 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
             X[a] = x; Y[a] = y; Z[a] = z;
             VX[a] = vx; VY[a] = vy; VZ[a] = vz;
         }
-        pos[sl] = slot_position<P>(x, y, z, Q[a]);
         const double dx = min_image1(x - XB[a], d.L[0]);
         const double dy = min_image1(y - YB[a], d.L[1]);
         const double dz = min_image1(z - ZB[a], d.L[2]);
+        if constexpr (P::mixed)
+            pos[sl] = slot_position<P>(XB[a] + dx, YB[a] + dy, ZB[a] + dz, Q[a]);     // continuous_position
+        else
+            pos[sl] = slot_position<P>(x, y, z, Q[a]);
         const double d2 = dx * dx + dy * dy + dz * dz;
         moved = !(d2 <= d.skin_half2);
     }
@@ -205,7 +210,8 @@ template <class P>
 void launch_pack(Batch<P>& B) {
     const int T = 256;
     pack_kernel<P><<<(unsigned)((B.n_slots_cap + T - 1) / T), T, 0, B.stream>>>(
-        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.X, B.Y, B.Z, B.q, B.pos, B.FSx, B.FSy, B.FSz, B.n_slots_cap);
+        B.d_sys, B.d_cnt, B.slot_sys, B.atom_of_slot, B.X, B.Y, B.Z, B.XB, B.YB, B.ZB, B.q, B.pos, B.FSx, B.FSy,
+        B.FSz, B.n_slots_cap);
 }
 
 template <class P>

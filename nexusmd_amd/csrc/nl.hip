@@ -35,7 +35,7 @@
 namespace nx {
 
 constexpr int kColSortCap = 2048;     // atoms per column sorted in LDS
-constexpr int kListCap = 1024;        // pair-list entries per super-cluster (LDS)
+constexpr int kListCap = kSuperListCap;   // pair-list entries per super-cluster (LDS)
 constexpr int kExclCap = 2048;        // exclusion triples per super-cluster (LDS)
 
 // Column of an atom from its wrapped position.
@@ -166,7 +166,7 @@ __global__ void nl_scatter(const SysDev* sd, SysCount* sc, const int* atom_sys, 
 
 // One block (256 threads) per column: deterministic z order, cluster boxes,
 // super items, and the slot-order data of the force kernels (pos: x, y, z, q;
-// NaN coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).
+// kPadPos coordinates and q = 0 on padding slots; lj: sigma/2, 2 sqrt(eps)).
 template <class P>
 __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc, const int* col_sys,
                                                   const int* col_count, int* col_fill, const int* col_slot,
@@ -234,29 +234,31 @@ __global__ __launch_bounds__(256) void nl_columns(const SysDev* sd, SysCount* sc
             float zk = (float)key[i];
             if ((double)zk > key[i]) zk = nextafterf(zk, -1.0f);
             slot_zkey[slot] = zk;
-            const double x = X[a], y = Y[a], z = Z[a];
             double w[3];
-            wrapped(d, x, y, z, w);
+            wrapped(d, X[a], Y[a], Z[a], w);
             slot_wpos[slot] = make_float4((float)w[0], (float)w[1], (float)w[2], 1.0f);
+            // the build position is the wrapped one; the mixed path's slot
+            // positions continue from it until the next build, the FP64 path's
+            // are the positions themselves (the reference's arithmetic)
             typename P::real4 p;
-            p.x = (real)x;
-            p.y = (real)y;
-            p.z = (real)z;
+            p.x = (real)(P::mixed ? w[0] : X[a]);
+            p.y = (real)(P::mixed ? w[1] : Y[a]);
+            p.z = (real)(P::mixed ? w[2] : Z[a]);
             p.w = (real)Q[a];
             pos[slot] = p;
             typename P::real2 l;
             l.x = (real)(0.5 * sig[a]);
             l.y = (real)(2.0 * sqrt(eps[a]));
             lj[slot] = l;
-            XB[a] = x;
-            YB[a] = y;
-            ZB[a] = z;
+            XB[a] = w[0];
+            YB[a] = w[1];
+            ZB[a] = w[2];
         } else {
             atom_of_slot[slot] = -1;
             const float nan = __int_as_float(0x7fc00000);     // compares false in every distance test
             slot_wpos[slot] = make_float4(nan, nan, nan, 0.f);
             typename P::real4 p;
-            p.x = p.y = p.z = (real)__builtin_nan("");
+            p.x = p.y = p.z = (real)kPadPos;
             p.w = 0;
             pos[slot] = p;
             typename P::real2 l;
@@ -314,6 +316,31 @@ __device__ __forceinline__ float bbox_gap2(float4 ca, float4 ha, float4 cb, floa
     return dx * dx + dy * dy + dz * dz;
 }
 
+// The periodic image (nx, ny, nz in -1..1) of cluster b as seen from cluster a,
+// from their wrapped build-time boxes, as the code (nx+1) + 3 (ny+1) + 9 (nz+1):
+// every atom pair (p in a, q in b) then has p - q - n L as its nearest image
+// vector until the next build, because |c_a - c_b - n L| + h_a + h_b + slack
+// < L/2 on every axis (slack: the atoms' motion until the build, and
+// round-off).  -1 when the boxes do not guarantee it.
+constexpr int kImgMixed = 27;      // the entry's tiles need different images
+__device__ __forceinline__ int tile_image(float4 ca, float4 ha, float4 cb, float4 hb, float Lx, float Ly, float Lz,
+                                          float iLx, float iLy, float iLz, float slack) {
+    const float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
+    const float nx = rintf(dx * iLx), ny = rintf(dy * iLy), nz = rintf(dz * iLz);
+    const bool ok = fabsf(dx - nx * Lx) + ha.x + hb.x + slack < 0.5f * Lx &&
+                    fabsf(dy - ny * Ly) + ha.y + hb.y + slack < 0.5f * Ly &&
+                    fabsf(dz - nz * Lz) + ha.z + hb.z + slack < 0.5f * Lz;
+    return ok ? (int)nx + 3 * (int)ny + 9 * (int)nz + 13 : -1;
+}
+// The list word's image bits (two per axis: 0 none, 1 +L, 2 -L) of an image code.
+__device__ __forceinline__ unsigned image_bits(int code) {
+    const int n[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+    unsigned b = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) b |= (n[ax] == 1 ? 1u : (n[ax] == -1 ? 2u : 0u)) << (2 * ax);
+    return b;
+}
+
 // Position in increasing index order of the k-th column of a periodic window of
 // nwin columns centred on c0 (nwin == nc: every column).  Enumerating windows
 // this way makes the cluster indices of the candidates increase, so the list
@@ -356,7 +383,9 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
 //   1. the columns of the xy window whose rectangle comes within rlist of the
 //      super-cluster's box, in increasing column order;
 //   2. their clusters as one flat candidate range, one thread per candidate:
-//      cluster box against each i-cluster box, ordered compaction;
+//      cluster box against each i-cluster box, ordered compaction; the
+//      entry's periodic image (tile_image), when one image holds for all of
+//      its tiles, else the per-pair marker;
 //   3. per entry and i-cluster, the 64 atom pairs at build positions, one
 //      wavefront per entry: a tile is kept only if one of its pairs is within
 //      rlist (both atoms move at most skin/2 before the next build, so every
@@ -385,6 +414,7 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
 
     __shared__ unsigned int l_ent[kListCap];      // cj | imask << 24
     __shared__ int l_eid[kListCap];
+    __shared__ unsigned char l_img[kListCap];     // periodic image code of the entry's j cluster
     __shared__ unsigned int x_rec[kExclCap];      // (entry * 8 + i-cluster) << 6 | pair bit
     __shared__ int n_x, id_base, lbase, cbase;
     __shared__ float4 ic[8], ih[8];
@@ -404,6 +434,9 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     const float Lx = (float)d.L[0], Ly = (float)d.L[1], Lz = (float)d.L[2];
     const float iLx = (float)d.invL[0], iLy = (float)d.invL[1], iLz = (float)d.invL[2];
     const float rl2 = (float)(d.rlist * d.rlist) * 1.0001f;
+    // what the tile image test adds to the cluster extents: the motion of two
+    // atoms until the next build (skin/2 each) and 1e-3 nm for float round-off
+    const float img_slack = (float)(d.rlist - d.rc) + 1e-3f;
 
     // xy box of the super-cluster
     float xlo = 1e30f, xhi = -1e30f, ylo = 1e30f, yhi = -1e30f;
@@ -451,7 +484,7 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         __syncthreads();
         for (int b = 0; b < total; b += kPlThreads) {
             const int t = b + tid;
-            unsigned int im = 0;
+            unsigned int im = 0, img = kImgPerPair;
             int cj = 0;
             if (t < total) {
                 int lo = 0, hi = kPlThreads - 1;      // last column whose range starts at or before t
@@ -461,13 +494,22 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
                 }
                 cj = w_first[lo] + (t - w_pre[lo]);
                 const float4 jc = cl_ctr[cj], jh = cl_half[cj];
+                int code = -1;
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (k < ncl && cj >= fc + k && bbox_gap2(ic[k], ih[k], jc, jh, Lx, Ly, Lz) < rl2) im |= 1u << k;
+                    if (k < ncl && cj >= fc + k && bbox_gap2(ic[k], ih[k], jc, jh, Lx, Ly, Lz) < rl2) {
+                        im |= 1u << k;
+                        const int ck = tile_image(ic[k], ih[k], jc, jh, Lx, Ly, Lz, iLx, iLy, iLz, img_slack);
+                        code = (ck < 0 || (code >= 0 && ck != code) || code == kImgMixed) ? kImgMixed : ck;
+                    }
+                img = (d.pair_image && code >= 0 && code != kImgMixed) ? image_bits(code) : kImgPerPair;
             }
             int cnt;
             const int pos = ne + block_excl_scan(im != 0, wsum, cnt);
-            if (im && pos < kListCap) l_ent[pos] = (unsigned)cj | (im << 24);
+            if (im && pos < kListCap) {
+                l_ent[pos] = (unsigned)cj | (im << 24);
+                l_img[pos] = (unsigned char)img;
+            }
             ne += cnt;
         }
         __syncthreads();
@@ -518,10 +560,11 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
         for (int b = 0; b < ne; b += kPlThreads) {
             const int e = b + tid;
             const unsigned int v = (e < ne) ? l_ent[e] : 0u;
+            const unsigned char vi = (e < ne) ? l_img[e] : (unsigned char)0;
             const bool kept = (v >> 24) != 0u;
             int cnt;
             const int pos = nk + block_excl_scan(kept, wsum, cnt);   // every read precedes every write
-            if (kept) l_ent[pos] = v;
+            if (kept) { l_ent[pos] = v; l_img[pos] = vi; }
             nk += cnt;
             __syncthreads();
         }
@@ -609,7 +652,8 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
     for (int e = tid; e < ne; e += kPlThreads) {
         uint2 v;
         v.x = l_ent[e];
-        v.y = (l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : 0xffffffffu;
+        v.y = ((l_eid[e] >= 0) ? (unsigned)(d.emask_off + id_base + l_eid[e]) : kNoMask) |
+              ((unsigned)l_img[e] << kImgShift);
         list[d.list_off + lbase + e] = v;
     }
     if (tid == 0) {

@@ -59,34 +59,36 @@ __device__ __forceinline__ MixedPair mixed_pair_const(const SysDev& d) {
 }
 
 // Mixed path: FP32 pair math.  Returns dE/dr / r (force = d * value on i).
+// With erfc(alpha r) = ex u (ex = exp(-(alpha r)^2), u the A&S polynomial in
+// t), the Coulomb part qq (cg ex + erfc / r) / r^2 is qq ex (cg + u / r) / r^2,
+// so that it joins the LJ part in one multiply-add before the common 1/r^2.
 __device__ __forceinline__ float pair_mixed(float r2, float qq, float sig, float epsij, const MixedPair& c,
                                             float& ecoul, float& elj) {
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
     const float inv_r2 = inv_r * inv_r;
     const float ex = __builtin_amdgcn_exp2f(c.ek * r2);                  // exp(-(alpha r)^2)
-    const float erfcv = erfc_as_t(__builtin_amdgcn_rcpf(fmaf(c.ap, r, 1.0f)), ex);
-    const float ce = qq * inv_r * erfcv;
-    const float dc = fmaf(qq * c.cg, ex, ce) * inv_r2;
+    const float u = erfc_as_t(__builtin_amdgcn_rcpf(fmaf(c.ap, r, 1.0f)), 1.0f);
+    const float cy = ex * fmaf(inv_r, u, c.cg);
     const float s2 = sig * sig * inv_r2;
     const float s6 = s2 * s2 * s2;
     float le = epsij * (s6 - 1.0f) * s6;
-    float dl = epsij * (12.0f * s6 - 6.0f) * s6 * inv_r2;
-    if (c.sw_on >= 0.f) {   // uniform per system; t = 0 inside r_switch gives sv = 1, sd = 0
+    float dl = epsij * (12.0f * s6 - 6.0f) * s6;                          // r^2 times its dE/dr / r
+    if (c.sw_on >= 0.f) {   // uniform per system
         // sv = 1 - 10 t^3 + 15 t^4 - 6 t^5 = w^3 (1 + 3 t + 6 t^2) with w = 1 - t,
         // dsv/dt = -30 t^2 w^2.  The factored form keeps FP32 accuracy where sv
         // is small (r near rc); w = 1 inside r_switch gives sv = 1, sd = 0.
         const float w = fminf((c.sw_rc - r) * c.sw_inv, 1.0f);
         const float t = 1.0f - w;
         const float sv = w * w * w * fmaf(t, fmaf(t, 6.0f, 3.0f), 1.0f);
-        const float u = t * w;
-        const float sd = c.sw_d * u * u;
-        dl = sv * dl - le * sd * inv_r;
+        const float uw = t * w;
+        const float sd = c.sw_d * uw * uw;
+        dl = sv * dl - le * sd * r;
         le *= sv;
     }
-    ecoul = ce;
+    ecoul = qq * inv_r * ex * u;
     elj = le;
-    return dc + dl;
+    return fmaf(qq, cy, dl) * inv_r2;
 }
 
 // FP64 path: the reference's formulas with library erfc/exp.
@@ -138,7 +140,7 @@ constexpr unsigned long long lower_triangle_mask() {
 constexpr unsigned long long kLowerTriangle = lower_triangle_mask();
 
 template <class P, bool ENERGY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_waves_per_simd))) void pair_kernel(const SysDev* __restrict__ sd, const int* __restrict__ prefix,
+__device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const int* __restrict__ prefix,
                                                    int nsys, const int* __restrict__ wave_start,
                                                    const int4* __restrict__ chunks,
                                                    const uint2* __restrict__ list,
@@ -217,10 +219,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
             q.p.x = q.p.y = q.p.z = q.p.w = 0;
             q.l.x = q.l.y = 0;
         }
-        q.ent = list[q.ch.z];
-        const int sj = (int)(q.ent.x & 0xffffffu) * 8 + ja;
-        q.pj = pos[sj];
-        q.ljj = lj[sj];
+        // the first entry and its j data (none for a chunk without entries)
+        q.ent = make_uint2(0u, kNoMask);
+        q.pj.x = q.pj.y = q.pj.z = q.pj.w = 0;
+        q.ljj.x = q.ljj.y = 0;
+        if (q.ch.w > q.ch.z) {
+            q.ent = list[q.ch.z];
+            const int sj = (int)(q.ent.x & 0xffffffu) * 8 + ja;
+            q.pj = pos[sj];
+            q.ljj = lj[sj];
+        }
         return q;
     };
 
@@ -265,16 +273,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
             // every lane loaded the same list word: move it to scalar registers so the
             // per-cluster tests below are scalar branches
             const unsigned ex = __builtin_amdgcn_readfirstlane(ent.x);
-            const unsigned eid = __builtin_amdgcn_readfirstlane(ent.y);
+            const unsigned ey = __builtin_amdgcn_readfirstlane(ent.y);
             const int cj = (int)(ex & 0xffffffu);
             const unsigned im = ex >> 24;
+            const unsigned eid = ey & kNoMask, img = ey >> kImgShift;
             // exclusion bits of the 8 cluster pairs, one load before the cluster loop
             unsigned long long em[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) em[k] = 0ull;
-            if (eid != 0xffffffffu) {
+            if (eid != kNoMask) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) em[k] = emask[(size_t)eid * 8 + k];
+            }
+            // mixed path: the j cluster at the entry's periodic image (the list build
+            // checked that it is the nearest image of every pair that can be within
+            // the cutoff), or, for an entry whose pairs need it, the nearest image
+            // pair by pair below; the FP64 path takes it pair by pair with the
+            // reference's arithmetic, so that its cutoff test is the reference's bit
+            // for bit.  Padding j slots move to -kPadPos, far from every i slot
+            // (padding ones at +kPadPos included).
+            const bool per_pair = !P::mixed || img == kImgPerPair;
+            real4 pjs;
+            {
+                const real sx = per_pair ? (real)0 : ((img & 3u) == 1u ? Lx : ((img & 3u) == 2u ? -Lx : (real)0));
+                const real sy = per_pair ? (real)0 : ((img & 12u) == 4u ? Ly : ((img & 12u) == 8u ? -Ly : (real)0));
+                const real sz = per_pair ? (real)0 : ((img & 48u) == 16u ? Lz : ((img & 48u) == 32u ? -Lz : (real)0));
+                pjs.x = (pj.x > (real)(0.5 * kPadPos)) ? (real)(-kPadPos) : pj.x + sx;
+                pjs.y = pj.y + sy;
+                pjs.z = pj.z + sz;
+                pjs.w = pj.w;
             }
             uint2 ent_n = ent;
             real4 pj_n = pj;
@@ -314,19 +341,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     // excluded lanes, as a wave-wide bit mask: listed exclusions, and on
                     // the diagonal tile the pairs with ja <= ia (self pairs and the other half)
                     const unsigned long long xm = em[k] | (cj == fc + k ? kLowerTriangle : 0ull);
-                    real dx, dy, dz;
-                    if constexpr (P::mixed) {
-                        // nearest image by rounding (equal to the one-correction rule for |d| < 1.5 L)
-                        dx = pi.x - pj.x;
-                        dy = pi.y - pj.y;
-                        dz = pi.z - pj.z;
-                        dx -= Lx * rintf(dx * iLx);
-                        dy -= Ly * rintf(dy * iLy);
-                        dz -= Lz * rintf(dz * iLz);
-                    } else {
-                        dx = min_image1(pi.x - pj.x, Lx);
-                        dy = min_image1(pi.y - pj.y, Ly);
-                        dz = min_image1(pi.z - pj.z, Lz);
+                    real dx = pi.x - pjs.x, dy = pi.y - pjs.y, dz = pi.z - pjs.z;
+                    if (per_pair) {     // a uniform branch
+                        if constexpr (P::mixed) {
+                            // nearest image by rounding (equal to the one-correction rule for
+                            // |d| < 1.5 L); a padding slot keeps its distance of ~kPadPos in x
+                            const real cx = dx - Lx * rintf(dx * iLx);
+                            dx = fabsf(dx) < (real)(0.5 * kPadPos) ? cx : dx;
+                            dy -= Ly * rintf(dy * iLy);
+                            dz -= Lz * rintf(dz * iLz);
+                        } else {
+                            dx = min_image1(dx, Lx);
+                            dy = min_image1(dy, Ly);
+                            dz = min_image1(dz, Lz);
+                        }
                     }
                     real r2;
                     if constexpr (P::mixed) r2 = dx * dx + dy * dy + dz * dz;
@@ -342,19 +370,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
                     const bool on = __builtin_amdgcn_inverse_ballot_w64(onm);
                     real ecv, elv, dEdR;
                     if constexpr (P::mixed) {
-                        // evaluated on every lane; lanes that are off (incl. r2 = 0 or NaN) select zero
-                        dEdR = pair_mixed(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, mpc, ecv, elv);
+                        // evaluated on every lane; lanes that are off (incl. r2 = 0) select zero
+                        dEdR = pair_mixed(r2, pi.w * pjs.w, li.x + lj_j.x, li.y * lj_j.y, mpc, ecv, elv);
                         dEdR = on ? dEdR : 0.0f;
                     } else {
                         dEdR = 0;
                         ecv = elv = 0;
                         if (on)
-                            dEdR = pair_fp64(r2, pi.w * pj.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
+                            dEdR = pair_fp64(r2, pi.w * pjs.w, li.x + lj_j.x, li.y * lj_j.y, alpha, sw_on,
                                              sw_inv, ecv, elv);
                     }
-                    // off lanes may hold NaN displacements (padding slots): select both factors
-                    const real fx = (on ? dx : (real)0) * dEdR, fy = (on ? dy : (real)0) * dEdR,
-                               fz = (on ? dz : (real)0) * dEdR;
+                    // every displacement is finite (padding slots included), so the
+                    // zero of an off lane gives zero force
+                    const real fx = dx * dEdR, fy = dy * dEdR, fz = dz * dEdR;
                     fxi[k] += fx; fyi[k] += fy; fzi[k] += fz;
                     fxj -= fx; fyj -= fy; fzj -= fz;
                     if constexpr (ENERGY) {
@@ -428,6 +456,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::pair_wav
     }
 }
 
+#define NX_PAIR_PARAMS(P)                                                                                      \
+    const SysDev *__restrict__ sd, const int *__restrict__ prefix, int nsys, const int *__restrict__ wave_start,  \
+        const int4 *__restrict__ chunks, const uint2 *__restrict__ list, const unsigned long long *__restrict__ emask, \
+        const typename P::real4 *__restrict__ pos, const typename P::real2 *__restrict__ lj, typename P::facc *FX,    \
+        typename P::facc *FY, typename P::facc *FZ, long long *energy, int *__restrict__ pool
+#define NX_PAIR_ARGS sd, prefix, nsys, wave_start, chunks, list, emask, pos, lj, FX, FY, FZ, energy, pool
+
+// The pair kernels.  The mixed one without energies must stay within
+// PrecMixed::pair_max_vgpr VGPRs (the compiler offers no cap short of a higher
+// occupancy target, which spills; a test of the build checks the count), so
+// that the PME kernels still find room beside its persistent grid.
+template <bool ENERGY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PrecMixed::pair_waves_per_simd))) void
+pair_kernel_mixed(NX_PAIR_PARAMS(PrecMixed)) {
+    pair_body<PrecMixed, ENERGY>(NX_PAIR_ARGS);
+}
+template <bool ENERGY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PrecFP64::pair_waves_per_simd))) void
+pair_kernel_fp64(NX_PAIR_PARAMS(PrecFP64)) {
+    pair_body<PrecFP64, ENERGY>(NX_PAIR_ARGS);
+}
+#undef NX_PAIR_ARGS
+#undef NX_PAIR_PARAMS
+
+template <class P, bool ENERGY>
+constexpr auto pair_kernel_of() {
+    if constexpr (P::mixed) return pair_kernel_mixed<ENERGY>;
+    else return pair_kernel_fp64<ENERGY>;
+}
+
 // Explicit pair set (atom ids, first > second) with the FP64 test of the pair kernel.
 template <class P>
 __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, const SysCount* sc, const int* super_sys,
@@ -453,7 +511,8 @@ __global__ __launch_bounds__(64) void export_pairs_kernel(const SysDev* sd, cons
             if (!(im & (1u << k))) continue;
             const int ai = atom_of_slot[(fc + k) * 8 + ia];
             bool excl = (cj == fc + k) && (ja <= ia);
-            if (ent.y != 0xffffffffu) excl |= ((emask[(size_t)ent.y * 8 + k] >> lane) & 1ull) != 0ull;
+            const unsigned eid = ent.y & kNoMask;
+            if (eid != kNoMask) excl |= ((emask[(size_t)eid * 8 + k] >> lane) & 1ull) != 0ull;
             if (excl || ai < 0 || aj < 0) continue;
             const double dx = min_image1(X[ai] - X[aj], Lx);
             const double dy = min_image1(Y[ai] - Y[aj], Ly);
@@ -471,7 +530,7 @@ template <class P>
 static int pair_blocks_per_cu() {
     static const int v = [] {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_kernel<P, false>, 256, 0) != hipSuccess || n <= 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_kernel_of<P, false>(), 256, 0) != hipSuccess || n <= 0)
             n = 2;
         return n;
     }();
@@ -512,11 +571,11 @@ void launch_pair(Batch<P>& B, bool energy) {
     if (B.n_chunk_cap == 0) return;
     const unsigned blocks = (unsigned)B.pair_blocks;
     if (energy)
-        pair_kernel<P, true><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
+        pair_kernel_of<P, true>()<<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
                                                            B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy,
                                                            B.pair_pool);
     else
-        pair_kernel<P, false><<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
+        pair_kernel_of<P, false>()<<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
                                                             B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy,
                                                             B.pair_pool);
 }

@@ -50,7 +50,8 @@ struct SysDev {
     int list_off, list_cap;
     int emask_off, emask_cap;
     double colw[2];              // column widths
-    int grid[3], pad0;
+    int grid[3];
+    int pair_image;              // mixed path: per-entry periodic images in the list (0: pair by pair)
     long long rgrid_off, cgrid_off;   // offsets into the real / complex grids
     int excl_off, n_excl;        // excluded pair list
     int bond_off, n_bonds, angle_off, n_angles, tors_off, n_tors;
@@ -67,6 +68,11 @@ struct SysCount {
 };
 
 constexpr int kChunkEntries = 32;       // pair-list entries per chunk, the pair kernel's unit of work
+// Pair-list entries one super-cluster may hold (the list build keeps them in
+// LDS).  A super-cluster is 64 atoms of one column, about 3.5 nm tall at water
+// density, with ~700-950 j clusters within rlist = 1.1 nm: 1024 was exceeded
+// by a few super-clusters of a 1M-atom water box.
+constexpr int kSuperListCap = 2048;
 constexpr int kPairWaves = 4;           // independent wavefronts per 256-thread pair-kernel block
 // Estimated pair-kernel work in quarter tiles: per tile, per list entry (the
 // j loads, the j-force reduction and its LDS staging) and per chunk (i staging
@@ -77,6 +83,22 @@ constexpr int kPairWaves = 4;           // independent wavefronts per 256-thread
 constexpr int kCostTile = 4, kCostEntry = 11, kCostChunk = 24;
 // The pair kernel's chunk counters, one per XCD, 128 bytes apart.
 constexpr int kPoolStride = 32, kPoolInts = 8 * kPoolStride;
+
+// Second word of a pair-list entry: the exclusion-mask id in the low 26 bits
+// (kNoMask: the tile has no listed exclusion) and the j-cluster's periodic
+// image in the high 6 bits, two per axis (0: none, 1: +L, 2: -L), for which
+// every atom pair of the entry that can come within the cutoff before the next
+// build is at its nearest image; kImgPerPair (all six bits set): the entry's
+// pairs need the nearest image pair by pair.
+constexpr unsigned kNoMask = 0x3ffffffu;
+constexpr unsigned kImgShift = 26u;
+constexpr unsigned kImgPerPair = 0x3fu;
+
+// Slot position of a padding slot.  Finite, so that the pair kernel's force
+// products stay finite, and far outside every box, so that a padding slot
+// fails every cutoff test; the pair kernel moves padding j slots to -kPadPos
+// so that two padding slots are far apart as well.
+constexpr double kPadPos = 1e15;
 
 // Issue priority of a wavefront of the kernels that run beside the pair kernel
 // (PME, bonded terms), above the pair kernel's default 0: the sequencer
@@ -102,6 +124,11 @@ struct PrecMixed {
     using facc = long long;       // 2^32 fixed point
     static constexpr bool mixed = true;
     static constexpr int pair_waves_per_simd = 4;
+    // Three pair-kernel waves per SIMD (the persistent grid's three blocks per
+    // CU) at 112 VGPRs leave 176 of a SIMD's 512 to the PME kernels beside
+    // them: a 1024-thread plane-FFT block (four waves per SIMD at 40 VGPRs)
+    // then fits; at 120 it does not, and PME waits for the pair kernel.
+    static constexpr int pair_max_vgpr = 112;
 };
 
 __device__ __forceinline__ long long to_fixed(float v) {
@@ -204,6 +231,19 @@ __device__ __forceinline__ float min_image1(float d, float L) {
     const float h = 0.5f * L;
     const float up = d - L, dn = d + L;
     return d > h ? up : (d < -h ? dn : d);
+}
+
+// The slot position the mixed path's pair and PME kernels read: the build-time
+// wrapped position plus the displacement since the build, continuous between
+// builds (an atom's wrapped position jumps by L when it crosses the box), so
+// that a j-cluster's periodic image found at the build holds until the next
+// one.  (The FP64 path reads the positions themselves and takes the nearest
+// image pair by pair with the reference's arithmetic.)
+__device__ __forceinline__ void continuous_position(const SysDev& d, double x, double y, double z, double xb,
+                                                    double yb, double zb, double* p) {
+    p[0] = xb + min_image1(x - xb, d.L[0]);
+    p[1] = yb + min_image1(y - yb, d.L[1]);
+    p[2] = zb + min_image1(z - zb, d.L[2]);
 }
 
 // r^2 without contraction, left-to-right as Coords3D::dot (Coords3D.h:116-124).
