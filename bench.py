@@ -49,7 +49,8 @@ PAIR_FLOP = 75           # FLOP per within-cutoff pair (FMA = 2, rsqrt/exp/rcp =
 PAIR_SAMPLE_STEPS = 64   # steps whose pair kernel is bracketed by events (beside PME)
 PAIR_ALONE_STEPS = 32    # steps of per-kernel events (every kernel family alone)
 STEADY_STEPS = 2000
-DRIFT_STEPS, DRIFT_EVERY = 5000, 250
+DRIFT_EQUIL, DRIFT_STEPS, DRIFT_EVERY = 4000, 5000, 250
+TUNE_STEPS = 50
 
 
 def log(*a):
@@ -148,6 +149,68 @@ def time_path(sysm, prec, steps, warmup, skin):
     return ctx, wall, kern, stats, 0.5 * (pairs0 + pairs1), pair
 
 
+def autotune_leg(sysm, prec, skin, steps_per_trial=TUNE_STEPS):
+    """nx_autotune on the system: the chosen launch shapes and the measured
+    ms/step of the default and of the choice."""
+    from nexusmd_amd import engine as E
+    ctx = E.Context(sysm, precision=prec, skin=skin)
+    t0 = time.perf_counter()
+    chosen, ms_default, ms_best = ctx.autotune(steps_per_trial)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    return {"atoms": sysm.n, "chosen": chosen, "ms_per_step_default": ms_default, "ms_per_step_chosen": ms_best,
+            "steps_per_trial": steps_per_trial, "tuning_s": sec}
+
+
+def ingest_bench(sysm):
+    """Host ingest as TaskDispatcher::Simulate runs it, stage by stage, on the
+    workload written as system/state XML (one report step, no timing of the
+    step itself): the system file in one pass, the state file, the topology
+    with its exclusions, box and state, and the device set-up."""
+    import tempfile
+    from nexusmd_amd import build as Bd
+    from nexusmd_amd import xmlio
+    sys.path.insert(0, os.path.dirname(Bd.module_path()))
+    import NexusMD
+    with tempfile.TemporaryDirectory() as d:
+        t = time.perf_counter()
+        xmlio.write_system_xml(sysm, os.path.join(d, "system.xml"))
+        xmlio.write_state_xml(sysm, os.path.join(d, "state.xml"))
+        xmlio.write_pdb(sysm, os.path.join(d, "input.pdb"))
+        written = time.perf_counter() - t
+        sizes = {f: os.path.getsize(os.path.join(d, f)) for f in ("system.xml", "state.xml")}
+        disp = NexusMD.TaskDispatcher.CreateDispatcher("GPU_MIXED")
+        disp.echo_steps = False
+        for en in ("EnableHarmonicBondForce", "EnableHarmonicAngleForce", "EnablePeriodicTorsionForce",
+                   "EnableNonbondedForce"):
+            getattr(disp, en)()
+        cwd = os.getcwd()
+        os.chdir(d)
+        # the dispatcher prints the reference's console lines; stdout carries
+        # only the bench's JSON line, so they go to stderr
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)
+        try:
+            disp.Simulate("system.xml", "state.xml", "input.pdb", "out.pdb", sysm.dt, 1, 1)
+        finally:
+            sys.stdout.flush()
+            os.dup2(saved, 1)
+            os.close(saved)
+            os.chdir(cwd)
+        times = dict(disp.IngestTimes())
+    times["total_s"] = sum(v for k, v in times.items() if k.endswith("_s"))
+    times["xml_bytes"] = sizes
+    times["xml_written_in_s"] = written
+    return times
+
+
+def serving_median_system():
+    """A ligand-in-water request of the serving mix's median size (2,500 atoms)."""
+    from nexusmd_amd import systems as S
+    return S.ligand_in_water(2500, 7)
+
+
 def steady_state(sysm, prec, skin, steps=STEADY_STEPS):
     """ns/day of `steps` graph-replayed steps after 100 warm-up steps."""
     from nexusmd_amd import engine as E
@@ -162,12 +225,17 @@ def steady_state(sysm, prec, skin, steps=STEADY_STEPS):
             "rebuilds": rebuilds}
 
 
-def energy_drift(sysm, prec, skin, steps=DRIFT_STEPS, every=DRIFT_EVERY):
+def energy_drift(sysm, prec, skin, steps=DRIFT_STEPS, every=DRIFT_EVERY, equil=DRIFT_EQUIL):
     """Total energy (kinetic with the reference's half-step shift + physical
-    potential) every `every` steps over `steps` steps; drift = least-squares
-    slope in kJ/mol per ns, also per atom."""
+    potential) every `every` steps over `steps` steps, after `equil` steps in
+    which the freshly thermalised system shares its energy between kinetic and
+    potential (the total is not flat there: the first picoseconds move a large
+    share of it between the stiff modes, where the step's integration error is
+    largest); drift = least-squares slope in kJ/mol per ns, also per atom."""
     from nexusmd_amd import engine as E
     ctx = E.Context(sysm, precision=prec, skin=skin)
+    if equil:
+        ctx.step(equil, sysm.dt)
     t, etot = [], []
     done = 0
     while done <= steps:
@@ -182,8 +250,8 @@ def energy_drift(sysm, prec, skin, steps=DRIFT_STEPS, every=DRIFT_EVERY):
     ctx.close()
     t, etot = np.asarray(t), np.asarray(etot)
     slope = float(np.polyfit(t, etot, 1)[0])
-    return {"steps": steps, "samples": len(t), "kj_mol_per_ns": slope, "kj_mol_per_ns_per_atom": slope / sysm.n,
-            "std_kj_mol": float(np.std(etot)), "e0_kj_mol": float(etot[0])}
+    return {"equilibration_steps": equil, "steps": steps, "samples": len(t), "kj_mol_per_ns": slope,
+            "kj_mol_per_ns_per_atom": slope / sysm.n, "std_kj_mol": float(np.std(etot)), "e0_kj_mol": float(etot[0])}
 
 
 def host_info():
@@ -429,7 +497,7 @@ def reports_bench(steps_c0=400, steps_c1=60, workdir=None):
 PROFILE = os.path.join(ROOT, "profiles", "kernels_solvated_chain_100000.md")
 
 
-def profiled_traffic(kernel="pair_kernel<mixed, false>"):
+def profiled_traffic(kernel="pair_kernel_mixed<false>"):
     """HBM bytes per launch of `kernel` as recorded by the two rocprofv3 --pmc
     passes (FETCH_SIZE, WRITE_SIZE, scaled by a 256 MiB copy) of this bench
     command, from the committed summary; None when it is absent."""
@@ -446,7 +514,7 @@ def profiled_traffic(kernel="pair_kernel<mixed, false>"):
 VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_pair_mixed_solvated_chain_100000.md")
 
 
-def profiled_valu_busy(kernel="pair_kernel<mixed, false>"):
+def profiled_valu_busy(kernel="pair_kernel_mixed<false>"):
     """Fraction of the kernel's cycles with a VALU instruction issuing
     (rocprofiler's VALUBusy from SQ_ACTIVE_INST_VALU and GRBM_GUI_ACTIVE), from
     the committed counter summary of this bench command; None when absent."""
@@ -487,6 +555,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-drift", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner leg (default launch shapes)")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the host-ingest leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving measurement")
     ap.add_argument("--serving-only", action="store_true", help="run only the batched-serving measurement")
     ap.add_argument("--reports-only", action="store_true",
@@ -542,6 +612,29 @@ def main():
                       "energy_rel": abs(e["potential"] - e_ref["potential"]) / abs(e_ref["potential"])}
         c.close()
     log("force errors", errs)
+
+    ingest = None
+    if not args.no_ingest:
+        try:
+            ingest = ingest_bench(sysm)
+            log("ingest", ingest)
+        except Exception as exc:          # reported, not fatal to the headline
+            ingest = {"error": str(exc)}
+
+    # launch shapes chosen by the autotuner (timed trials, outside every timed
+    # region); the engine keeps the choice per system shape for the process, so
+    # the timed paths below run with it
+    tuned = {}
+    if not args.no_tune:
+        for name, prec in (("mixed", L.MIXED), ("fp64", L.FP64)):
+            if name == "fp64" and args.no_fp64:
+                continue
+            tuned[name] = autotune_leg(sysm, prec, args.skin)
+        try:
+            tuned["serving_median_request"] = autotune_leg(serving_median_system(), L.MIXED, args.skin)
+        except Exception as exc:          # reported, not fatal to the headline
+            tuned["serving_median_request"] = {"error": str(exc)}
+        log("autotune", tuned)
 
     ctx_m, sec_m, kern_m, stats_m, pairs, pair_m = time_path(sysm, L.MIXED, args.steps, args.warmup, args.skin)
     ctx_m.close()
@@ -625,6 +718,8 @@ def main():
         "fp64": fp64,
         "mixed_over_fp64": (nsd_m / fp64["ns_per_day"]) if fp64 else None,
         "steady_state": steady,
+        "autotune": tuned or None,
+        "ingest": ingest,
         "energy_drift": drift,
         "force_error_vs_fp64_oracle": errs,
         "roofline": roofline,

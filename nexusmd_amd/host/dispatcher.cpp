@@ -7,6 +7,7 @@
 // nx_step_report (forces, energies, kinetic energy with the half-step shift
 // and the state before the update); the steps in between run as one nx_step
 // call (graph replays on the device, no host round trip per step).
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -240,7 +241,13 @@ void GpuTaskDispatcher::Simulate(const std::string& systemFilename, const std::s
     if (TotalSteps < 0) throw std::invalid_argument("TotalSteps must be >= 0");
     if (interval <= 0) throw std::invalid_argument("interval must be positive");
 
+    lastIngest = IngestTimes{};
+    using clock = std::chrono::steady_clock;
+    auto since = [](clock::time_point t) { return std::chrono::duration<double>(clock::now() - t).count(); };
+    auto t_stage = clock::now();
     SystemDescription sys = SystemXMLParser::Parse(systemFilename);
+    lastIngest.parse_system = since(t_stage);
+    t_stage = clock::now();
     const bool extended = useStateVelocities || useStateBoxVectors || restart;
     StateData state;
     if (extended) {
@@ -249,6 +256,7 @@ void GpuTaskDispatcher::Simulate(const std::string& systemFilename, const std::s
     } else {
         atomPositions = StateXMLParser::parseStateXML(stateFilename);
     }
+    lastIngest.parse_state = since(t_stage);
     masses = sys.masses;
     bondParams = sys.bonds;
     angleParams = sys.angles;
@@ -311,7 +319,10 @@ void GpuTaskDispatcher::Simulate(const std::string& systemFilename, const std::s
 
     Ctx ctx;
     check(nx_create(device, precision_, &ctx.h), nullptr, "nx_create");
+    t_stage = clock::now();
     check(nx_set_topology(ctx.h, &top), ctx.h, "nx_set_topology");
+    lastIngest.topology = since(t_stage);
+    t_stage = clock::now();
     double lb[3], ub[3];
     if (restart && state.hasBoxBounds) {
         // the exact bounds the checkpointed run used
@@ -345,6 +356,14 @@ void GpuTaskDispatcher::Simulate(const std::string& systemFilename, const std::s
             check(nx_set_list_state(ctx.h, bp.data(), state.rebuildDue), ctx.h, "nx_set_list_state");
         }
     }
+    lastIngest.box_state = since(t_stage);
+    t_stage = clock::now();
+    {
+        nx_stats st;      // lays the system out on the device: upload, first list build, PME tables
+        check(nx_get_stats(ctx.h, &st), ctx.h, "nx_get_stats");
+    }
+    lastIngest.device_setup = since(t_stage);
+    lastIngest.atoms = n;
     // a restartable state file after `done` steps
     auto checkpoint = [&](long long done) {
         StateData s;

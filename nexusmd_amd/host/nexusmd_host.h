@@ -214,6 +214,16 @@ class TaskDispatcher {
     std::string checkpointFilename;
     int checkpointInterval = 0;
 
+    // Wall-clock seconds of the last Simulate's ingest, stage by stage: the
+    // system XML (one pass), the state XML, the topology (validation and the
+    // 1-2/1-3/1-4 exclusions), box and state, and the device set-up (upload,
+    // first neighbour-list build, PME tables).
+    struct IngestTimes {
+        double parse_system = 0, parse_state = 0, topology = 0, box_state = 0, device_setup = 0;
+        long long atoms = 0;
+    };
+    IngestTimes lastIngest;
+
    protected:
     virtual void initializeParameters();
     std::vector<Coords3D> atomPositions;

@@ -31,7 +31,18 @@ PYBIND11_MODULE(NexusMD, m) {
         .def_readwrite("device", &TaskDispatcher::device)
         .def_readwrite("skin", &TaskDispatcher::skin)
         .def_readwrite("echo_steps", &TaskDispatcher::echo_steps)
-        .def_readwrite("async_reports", &TaskDispatcher::async_reports);
+        .def_readwrite("async_reports", &TaskDispatcher::async_reports)
+        .def("IngestTimes", [](const TaskDispatcher& d) {
+            const auto& t = d.lastIngest;
+            py::dict r;
+            r["atoms"] = t.atoms;
+            r["parse_system_s"] = t.parse_system;
+            r["parse_state_s"] = t.parse_state;
+            r["topology_s"] = t.topology;
+            r["box_state_s"] = t.box_state;
+            r["device_setup_s"] = t.device_setup;
+            return r;
+        });
 
     // Coords3D with the reference's operations (SWIG exports it, NexusMD.i:18)
     py::class_<Coords3D>(m, "Coords3D")

@@ -52,9 +52,12 @@ def _check_forces(sys, prec, ref, skin=0.1, label=""):
     return err
 
 
-def _drift(sys, prec, steps, every, dt):
-    """Total energy every `every` steps; least-squares slope per ns."""
+def _drift(sys, prec, steps, every, dt, equil=0):
+    """Total energy every `every` steps after `equil` steps; least-squares
+    slope per ns, the standard deviation, the mean and the list builds."""
     ctx = E.Context(sys, precision=prec, skin=0.1)
+    if equil:
+        ctx.step(equil, dt)
     t, tot = [], []
     done = 0
     while True:
@@ -90,17 +93,24 @@ def test_c1_water24000_forces_vs_oracle(c1, prec):
 
 
 def test_c1_water24000_drift_fp64_vs_mixed(c1):
-    """2,000 steps of 0.5 fs in both paths: the mixed path's total-energy drift
-    per ns within 30% of the FP64 path's (plus 2 kJ/mol/ns), its fluctuation
-    within 50%, and the FP64 drift below 1% of the total energy per ns."""
+    """Energy conservation of both paths over 2,000 steps of 0.5 fs, after 4,000
+    steps of equilibration: the jittered lattice starts far from equilibrium
+    (its kinetic energy triples in the first picosecond, a transient in which
+    the total moves by ~1e-3 in both paths alike).  In the window: the FP64
+    total fluctuates by at most 1e-4 of itself and drifts by no more than
+    three fluctuations; the mixed path's fluctuation is within 50% of the FP64
+    path's and its drift over the window within three FP64 fluctuations of the
+    FP64 drift."""
     s, _ = c1
-    sf, df, mf, rf = _drift(s, L.FP64, 2000, 100, s.dt)
-    sm, dm, mm, rm = _drift(s, L.MIXED, 2000, 100, s.dt)
-    print("[gpu] c1 drift kJ/mol/ns fp64 %.3f mixed %.3f (per atom %.2e, %.2e); std fp64 %.2f mixed %.2f; "
-          "rebuilds %d / %d" % (sf, sm, sf / s.n, sm / s.n, df, dm, rf, rm), flush=True)
-    assert abs(sf) <= 0.01 * abs(mf)
-    assert abs(sm - sf) <= 0.3 * abs(sf) + 2.0
+    T = 2000 * s.dt * 1e-3          # ns
+    sf, df, mf, rf = _drift(s, L.FP64, 2000, 100, s.dt, equil=4000)
+    sm, dm, mm, rm = _drift(s, L.MIXED, 2000, 100, s.dt, equil=4000)
+    print("[gpu] c1 drift after 2 ps, kJ/mol/ns: fp64 %.1f mixed %.1f (per atom %.2e, %.2e); std fp64 %.2f "
+          "mixed %.2f kJ/mol; rebuilds %d / %d" % (sf, sm, sf / s.n, sm / s.n, df, dm, rf, rm), flush=True)
+    assert df <= 1e-4 * abs(mf)
+    assert abs(sf) * T <= 3.0 * df
     assert dm <= 1.5 * df + 1.0
+    assert abs(sm - sf) * T <= 3.0 * df
 
 
 # ------------------------------------------------------------------- c2
@@ -196,24 +206,33 @@ def test_c4_fft_192_vs_numpy(prec, passes, monkeypatch):
 
 @pytest.mark.timeout(900)
 def test_c4_water_1m_100_steps(c4):
-    """100 steps of 0.5 fs with the 0.1 nm buffer in the mixed path: the list is
-    rebuilt on the device as atoms move (the count is printed beside the
-    configured cadence of every 20 steps), the total energy holds, and the
-    forces at the final positions, through the rebuilt lists, match the
-    oracle's."""
+    """100 steps of 0.5 fs with the 0.1 nm buffer: the list is rebuilt on the
+    device as atoms move (the count is printed beside the configured cadence of
+    every 20 steps), and the forces at the final positions, through the rebuilt
+    lists, match the oracle's.  The lattice starts far from equilibrium (its
+    kinetic energy grows 2.5-fold in these 50 fs), so the total energy moves by
+    a few 1e-3 through the step's integration error; the mixed path's change
+    is held to the FP64 path's within 2%."""
     s, _ = c4
-    ctx = E.Context(s, precision=L.MIXED, skin=0.1)
-    _, _, _, e0 = ctx.step_report(s.dt)
-    ctx.step(98, s.dt)
-    _, _, _, e1 = ctx.step_report(s.dt)
+
+    def run(prec):
+        ctx = E.Context(s, precision=prec, skin=0.1)
+        _, _, _, e0 = ctx.step_report(s.dt)
+        ctx.step(98, s.dt)
+        _, _, _, e1 = ctx.step_report(s.dt)
+        return ctx, e0["kinetic"] + e0["potential"], e1["kinetic"] + e1["potential"]
+
+    ctx64, f0, f1 = run(L.FP64)
+    ctx64.close()
+    ctx, t0, t1 = run(L.MIXED)
     rebuilds = ctx.stats()["rebuilds"]
     pos, vel = ctx.get_state()
-    t0, t1 = e0["kinetic"] + e0["potential"], e1["kinetic"] + e1["potential"]
-    print("[gpu] c4 100 steps: total energy %.1f -> %.1f kJ/mol, %d list builds (configured cadence: every 20 steps)"
-          % (t0, t1, rebuilds), flush=True)
+    print("[gpu] c4 100 steps: total energy mixed %.1f -> %.1f, fp64 %.1f -> %.1f kJ/mol, %d list builds "
+          "(configured cadence: every 20 steps)" % (t0, t1, f0, f1, rebuilds), flush=True)
     assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
     assert rebuilds >= 2
-    assert abs(t1 - t0) <= 1e-3 * abs(t0)
+    assert abs(f1 - f0) <= 5e-3 * abs(f0)
+    assert abs((t1 - t0) - (f1 - f0)) <= 0.02 * abs(f1 - f0) + 1e-6 * abs(f0)
     f, _ = ctx.compute_forces(L.ALL)
     ctx.close()
     s2 = s.copy()
@@ -222,3 +241,41 @@ def test_c4_water_1m_100_steps(c4):
     err = rel_rms(f, fr)
     print("[gpu] c4 after 100 steps: mixed force rel RMS %.3e" % err, flush=True)
     assert err <= TOL[L.MIXED]
+
+
+@pytest.mark.timeout(900)
+def test_c4_water_1m_ingest(c4, tmp_path):
+    """Host ingest at 1,000,002 atoms as TaskDispatcher::Simulate runs it: the
+    system XML in one pass, the state XML, the topology (validation and the
+    bonded exclusions), box and state, and the device set-up (upload, first
+    list build, PME tables); each stage is timed and printed, and the run's
+    first report step must complete (its energy row written)."""
+    import os
+    import sys
+    from nexusmd_amd import build as Bd
+    from nexusmd_amd import xmlio
+    sys.path.insert(0, os.path.dirname(Bd.module_path()))
+    import NexusMD
+    s, _ = c4
+    xmlio.write_system_xml(s, str(tmp_path / "system.xml"))
+    xmlio.write_state_xml(s, str(tmp_path / "state.xml"))
+    xmlio.write_pdb(s, str(tmp_path / "input.pdb"))
+    disp = NexusMD.TaskDispatcher.CreateDispatcher("GPU_MIXED")
+    disp.echo_steps = False
+    for en in ("EnableHarmonicBondForce", "EnableHarmonicAngleForce", "EnablePeriodicTorsionForce",
+               "EnableNonbondedForce"):
+        getattr(disp, en)()
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        print("[gpu] c4 ingest: Simulate 1 step from XML", flush=True)
+        disp.Simulate("system.xml", "state.xml", "input.pdb", "out.pdb", s.dt, 1, 1)
+    finally:
+        os.chdir(cwd)
+    t = disp.IngestTimes()
+    mb = os.path.getsize(tmp_path / "system.xml") / 1e6
+    print("[gpu] c4 ingest (%.0f MB system XML): %s" % (mb, {k: round(v, 3) for k, v in t.items()}), flush=True)
+    assert t["atoms"] == s.n
+    assert all(t[k] >= 0 for k in t)
+    # the report of step 0: header and one row
+    assert len(open(tmp_path / "TotalEnergyout.pdb").read().strip().splitlines()) == 2
