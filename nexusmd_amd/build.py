@@ -64,9 +64,11 @@ def build_host(verbose: bool = False) -> str:
     os.replace(tmp, HOST_LIB)
     mod = module_path()
     tmp = mod + ".tmp"
+    # the module finds the libraries beside it (in-tree) or in nexusmd_amd/ beside
+    # it (installed from the wheel, where NexusMD sits at the top level)
     _run([CXX, *HOST_FLAGS, "-fvisibility=hidden", "-I" + pybind11.get_include(),
           "-I" + sysconfig.get_paths()["include"], "-shared", "-o", tmp, os.path.join(HOST_DIR, "pymodule.cpp"),
-          "-L" + HERE, "-l:libnexusmd_host.so", *link], "Python module")
+          "-L" + HERE, "-l:libnexusmd_host.so", *link, "-Wl,-rpath,$ORIGIN/nexusmd_amd"], "Python module")
     os.replace(tmp, mod)
     if verbose:
         print("built", HOST_LIB)
