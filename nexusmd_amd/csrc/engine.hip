@@ -837,7 +837,8 @@ class Engine : public EngineBase {
         if (parts == PART_ALL && use_side(mask)) {
             fork_side();
             on_side([&] { side_work(mask, energy); });
-            timed("pair", [&] { launch_pair<P>(B, energy); });
+            launch_pair_begin<P>(B);
+            timed("pair", [&] { launch_pair_kernel<P>(B, energy); });
             join_side();
             return;
         }
@@ -845,7 +846,10 @@ class Engine : public EngineBase {
             if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });
             timed("atom_forces", [&] { launch_atom_forces<P>(B, mask, energy); });
         }
-        if (nb && (parts & PART_PAIR)) timed("pair", [&] { launch_pair<P>(B, energy); });
+        if (nb && (parts & PART_PAIR)) {
+            launch_pair_begin<P>(B);
+            timed("pair", [&] { launch_pair_kernel<P>(B, energy); });
+        }
         if (nb && (parts & PART_PME)) pme(energy);
     }
     void forces(int mask, bool energy) { forces_part(mask, energy, PART_ALL); }
@@ -931,6 +935,7 @@ class Engine : public EngineBase {
                 } else {
                     HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
                 }
+                if (seg_exec_[SEG_BEGIN]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_BEGIN], B.stream));
                 HIPCHK(hipEventRecord(a, B.stream));
                 if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
                 HIPCHK(hipEventRecord(b, B.stream));
@@ -1014,12 +1019,13 @@ class Engine : public EngineBase {
         B.timing = 0;
         capturing_ = true;
         for (int g = 0; g < SEG_COUNT; ++g) {
-            if (g == SEG_PAIR && !(mask & NX_FORCE_NONBONDED)) continue;     // no pair graph
+            if ((g == SEG_PAIR || g == SEG_BEGIN) && !(mask & NX_FORCE_NONBONDED)) continue;     // no pair graphs
             HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
             if (g == SEG_LAST) launch_integrate<P>(B, dt);
             if (g == SEG_FIRST) forces_part(mask, false, PART_LIST);
             if (g == SEG_SIDE) side_work(mask, false);
-            if (g == SEG_PAIR) launch_pair<P>(B, false);
+            if (g == SEG_BEGIN) launch_pair_begin<P>(B);
+            if (g == SEG_PAIR) launch_pair_kernel<P>(B, false);
             hipError_t e = hipStreamEndCapture(B.stream, &seg_[g]);
             if (e != hipSuccess) { capturing_ = false; B.timing = saved; HIPCHK(e); }
             HIPCHK(hipGraphInstantiate(&seg_exec_[g], seg_[g], nullptr, nullptr, 0));
@@ -1482,7 +1488,7 @@ class Engine : public EngineBase {
     std::vector<int> failed_sys_;           // systems whose list overflowed in the last call
     int grow_depth_ = 0;
     // timing segments: [list, positions], [terms, PME], [pair], [update]
-    enum { SEG_FIRST, SEG_SIDE, SEG_PAIR, SEG_LAST, SEG_COUNT };
+    enum { SEG_FIRST, SEG_SIDE, SEG_BEGIN, SEG_PAIR, SEG_LAST, SEG_COUNT };
     hipGraph_t seg_[SEG_COUNT] = {};
     hipGraphExec_t seg_exec_[SEG_COUNT] = {};
     hipStream_t side_ = nullptr;            // PME beside the pair kernel

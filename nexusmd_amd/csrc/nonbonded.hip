@@ -566,8 +566,12 @@ __global__ void pair_begin_kernel(SysCount* sc, int nsys, int* pool) {
 }
 
 template <class P>
-void launch_pair(Batch<P>& B, bool energy) {
+void launch_pair_begin(Batch<P>& B) {
     pair_begin_kernel<<<(B.nsys + 63) / 64, 64, 0, B.stream>>>(B.d_cnt, B.nsys, B.pair_pool);
+}
+
+template <class P>
+void launch_pair_kernel(Batch<P>& B, bool energy) {
     if (B.n_chunk_cap == 0) return;
     const unsigned blocks = (unsigned)B.pair_blocks;
     if (energy)
@@ -578,6 +582,12 @@ void launch_pair(Batch<P>& B, bool energy) {
         pair_kernel_of<P, false>()<<<blocks, 256, 0, B.stream>>>(B.d_sys, B.chunk_prefix, B.nsys, B.wave_start, B.chunks, B.list,
                                                             B.emask, B.pos, B.lj, B.FSx, B.FSy, B.FSz, B.energy,
                                                             B.pair_pool);
+}
+
+template <class P>
+void launch_pair(Batch<P>& B, bool energy) {
+    launch_pair_begin<P>(B);
+    launch_pair_kernel<P>(B, energy);
 }
 
 template <class P>
@@ -599,6 +609,10 @@ template int pair_blocks_occupancy<PrecFP64>();
 template int pair_blocks_occupancy<PrecMixed>();
 template void launch_pair<PrecFP64>(Batch<PrecFP64>&, bool);
 template void launch_pair<PrecMixed>(Batch<PrecMixed>&, bool);
+template void launch_pair_begin<PrecFP64>(Batch<PrecFP64>&);
+template void launch_pair_begin<PrecMixed>(Batch<PrecMixed>&);
+template void launch_pair_kernel<PrecFP64>(Batch<PrecFP64>&, bool);
+template void launch_pair_kernel<PrecMixed>(Batch<PrecMixed>&, bool);
 template void launch_export_pairs<PrecFP64>(Batch<PrecFP64>&, int2*, long long, long long*);
 template void launch_export_pairs<PrecMixed>(Batch<PrecMixed>&, int2*, long long, long long*);
 

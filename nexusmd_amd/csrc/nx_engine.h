@@ -137,6 +137,9 @@ template <class P> int pair_grid_blocks(int n_cu, int n_chunk_cap, int per_cu_re
 template <class P> int pair_blocks_occupancy();     // pair-kernel blocks one CU holds at once
 template <class P> void launch_pack(Batch<P>& B);
 template <class P> void launch_pair(Batch<P>& B, bool energy);
+// the two launches of launch_pair: the per-system set-up kernel, then the pair kernel
+template <class P> void launch_pair_begin(Batch<P>& B);
+template <class P> void launch_pair_kernel(Batch<P>& B, bool energy);
 template <class P> void launch_bonded(Batch<P>& B, int mask, bool energy);
 template <class P> void launch_atom_forces(Batch<P>& B, int mask, bool energy);
 template <class P> void launch_pme_spread(Batch<P>& B);
