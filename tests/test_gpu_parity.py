@@ -330,3 +330,35 @@ def test_switch_distance_is_validated():
             E.Context(s, precision=L.MIXED, use_switch=1, switch_distance=sd)
         assert ex.value.status == L.NX_ERR_ARG
     E.Context(s, precision=L.MIXED, use_switch=1, switch_distance=0.0).close()
+
+
+def test_pair_images_match_per_pair(monkeypatch):
+    """The mixed path's per-entry periodic images against the nearest image
+    taken pair by pair (NEXUSMD_PAIR_IMAGE=0), on a 4.2 nm water box where
+    most entries carry an image and the rest do not, after 300 steps so that
+    atoms have crossed the box faces since the list builds: the two differ
+    only by FP32 rounding of the displacements, and both match the oracle."""
+    print("[gpu] pair images vs per pair", flush=True)
+    s = _jiggle(S.water_box(n_mol=2500, L=4.217, grid=(40, 40, 40), seed=21), 3)
+    ctx = _ctx(s, L.MIXED, skin=0.1)
+    ctx.step(300, s.dt)
+    pos, _ = ctx.get_state()
+    f_img, e_img = ctx.compute_forces(L.ALL)
+    rebuilds = ctx.stats()["rebuilds"]
+    ctx.close()
+    s2 = s.copy()
+    s2.pos = pos
+    monkeypatch.setenv("NEXUSMD_PAIR_IMAGE", "0")
+    ctx = _ctx(s2, L.MIXED, skin=0.1)
+    f_pp, e_pp = ctx.compute_forces(L.ALL)
+    ctx.close()
+    fr, er = O.compute_forces(O.Prepared(s2))
+    d = rel_rms(f_img, f_pp)
+    print("[gpu] images vs per pair: rel RMS %.3e after %d builds; vs oracle %.3e / %.3e"
+          % (d, rebuilds, rel_rms(f_img, fr), rel_rms(f_pp, fr)), flush=True)
+    assert rebuilds >= 2
+    # FP32 rounding of the displacements, amplified by the steep LJ core (a
+    # wrong image would differ by O(1))
+    assert d <= 1e-4
+    assert rel_rms(f_img, fr) <= MIXED_TOL and rel_rms(f_pp, fr) <= MIXED_TOL
+    assert abs(e_img["potential"] - e_pp["potential"]) <= 1e-5 * abs(er["potential"])
