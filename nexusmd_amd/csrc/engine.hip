@@ -823,24 +823,31 @@ class Engine : public EngineBase {
     }
 
     // The parts in `parts`.  With every part and the side stream in use, the
-    // terms and PME go to the side stream after the slot positions and run
-    // beside the pair kernel (the pair kernel and PME add to the slot forces
-    // with atomics, the terms write the atom forces); the main stream then
-    // waits for the side stream.
+    // terms and PME go to the side stream once the list work has laid out the
+    // slots (before its last kernel, the chunk partition, which PME does not
+    // read) and run beside the pair kernel (the pair kernel and PME add to the
+    // slot forces with atomics, the terms write the atom forces); the main
+    // stream then waits for the side stream.
     void forces_part(int mask, bool energy, int parts) {
         const bool nb = (mask & NX_FORCE_NONBONDED) != 0;
+        if (parts == PART_ALL && use_side(mask)) {
+            // the side stream forks before the list's last kernel, which then does the
+            // pair kernel's set-up too: one launch fewer on the main stream (0.240 ->
+            // 0.237 ms per 100k-atom mixed step), and the side kernels still reach the
+            // CUs before the pair kernel's persistent grid
+            timed("nlist", [&] { launch_nlist<P>(B, !nb, 1); });
+            settle();
+            fork_side();
+            on_side([&] { side_work(mask, energy); });
+            timed("nlist", [&] { launch_nlist<P>(B, !nb, 2); });
+            timed("pair", [&] { launch_pair_kernel<P>(B, energy); });
+            join_side();
+            return;
+        }
         if (parts & PART_LIST) {
             // the pair kernel's first launch clears the rebuild flags
             timed("nlist", [&] { launch_nlist<P>(B, !nb); });
             settle();
-        }
-        if (parts == PART_ALL && use_side(mask)) {
-            fork_side();
-            on_side([&] { side_work(mask, energy); });
-            launch_pair_begin<P>(B);
-            timed("pair", [&] { launch_pair_kernel<P>(B, energy); });
-            join_side();
-            return;
         }
         if (parts & PART_TERMS) {
             if (mask & 7) timed("bonded", [&] { launch_bonded<P>(B, mask, energy); });

@@ -680,8 +680,21 @@ __global__ __launch_bounds__(kPlThreads) void nl_pairlist(const SysDev* sd, SysC
 // the total.
 constexpr int kWavesLdsSys = 1024;     // systems whose chunk prefix nl_waves keeps in LDS
 
+// With `pool` non-null it also does what the pair kernel's set-up launch does
+// (after every read of the flags): clears the rebuild flags, counts the
+// builds and resets the pair kernel's claim counters.
+__device__ __forceinline__ void waves_pair_begin(SysCount* sc, int nsys, int* pool) {
+    const int tid = threadIdx.x;
+    if (tid < 8) pool[tid * kPoolStride] = 0;
+    for (int s = tid; s < nsys; s += blockDim.x)
+        if (sc[s].rebuild) {
+            sc[s].rebuild = 0;
+            sc[s].rebuilds += 1;
+        }
+}
+
 __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost,
-                                                 int* prefix, int* wave_start, int NW) {
+                                                 int* prefix, int* wave_start, int NW, int* pool) {
     __shared__ int any, total;
     __shared__ long long wsum[16];
     __shared__ int pre_s[kWavesLdsSys + 1], off_s[kWavesLdsSys];
@@ -705,6 +718,7 @@ __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc,
         any = a;
     }
     __syncthreads();
+    if (pool) waves_pair_begin(sc, nsys, pool);       // the flags were read above
     if (!any) return;
     const int T = total;
     const bool lds = nsys <= kWavesLdsSys;
@@ -779,10 +793,15 @@ __global__ void nl_clear_flag(SysCount* sc, int nsys) {
 }
 
 template <class P>
-void launch_nlist(Batch<P>& B, bool clear_flags) {
+void launch_nlist(Batch<P>& B, bool clear_flags, int part) {
     const int T = 256;
     hipStream_t st = B.stream;
     auto nb = [](long n, int t) { return (unsigned)((n + t - 1) / t); };
+    if (part == 2) {       // the last kernel alone, doing the pair kernel's set-up too
+        nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves,
+                                     B.pair_pool);
+        return;
+    }
     nl_bin<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.atom_col, B.col_tmp,
                                           B.n_atoms);
     nl_scan<<<B.nsys, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.col_tmp, B.col_count, B.col_slot, B.col_super);
@@ -795,11 +814,13 @@ void launch_nlist(Batch<P>& B, bool clear_flags) {
     nl_pairlist<<<B.n_super_cap, kPlThreads, 0, st>>>(B.d_sys, B.d_cnt, B.super_sys, B.super_item, B.col_count, B.col_slot,
                                               B.cl_ctr, B.cl_half, B.slot_wpos, B.atom_of_slot, B.slot_of_atom, B.excl_off,
                                               B.excl_idx, B.list, B.emask, B.chunks, B.chunk_cost, B.n_super_cap);
-    nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves);
+    if (part == 1) return;  // all but the last kernel
+    nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves,
+                                 nullptr);
     if (clear_flags) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
 }
 
-template void launch_nlist<PrecFP64>(Batch<PrecFP64>&, bool);
-template void launch_nlist<PrecMixed>(Batch<PrecMixed>&, bool);
+template void launch_nlist<PrecFP64>(Batch<PrecFP64>&, bool, int);
+template void launch_nlist<PrecMixed>(Batch<PrecMixed>&, bool, int);
 
 }  // namespace nx

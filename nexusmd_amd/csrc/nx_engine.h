@@ -131,7 +131,9 @@ struct Batch {
     int timing = 0;                 // NX_TIMING_* mode
 };
 
-template <class P> void launch_nlist(Batch<P>& B, bool clear_flags);
+// part 0: the whole list work; 1: all but its last kernel; 2: the last kernel
+// alone, which then also does the pair kernel's set-up (launch_pair_begin)
+template <class P> void launch_nlist(Batch<P>& B, bool clear_flags, int part = 0);
 // persistent pair-kernel grid: per_cu_request blocks per CU (0: the built-in count), capped by occupancy
 template <class P> int pair_grid_blocks(int n_cu, int n_chunk_cap, int per_cu_request);
 template <class P> int pair_blocks_occupancy();     // pair-kernel blocks one CU holds at once
