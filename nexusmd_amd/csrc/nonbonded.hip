@@ -160,11 +160,11 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
     __shared__ real4 ipos_s[kPairWaves][64];
     __shared__ real2 ilj_s[kPairWaves][64];
     __shared__ real jf_s[kPairWaves][kChunkEntries * 3 * 8];
-    __shared__ int jc_s[kPairWaves][kChunkEntries];
+    __shared__ uint2 jw_s[kPairWaves][kChunkEntries];
     real4* ipos = ipos_s[w];
     real2* ilj = ilj_s[w];
     real* jf = jf_s[w];
-    int* jc = jc_s[w];
+    uint2* jw = jw_s[w];
 
     // The list build cuts the batch's chunks into one contiguous range of equal
     // estimated work per wavefront.  The wavefronts of one XCD (blocks b with
@@ -199,6 +199,7 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
         int4 ch;
         real4 p;
         real2 l;
+        uint2 words;     // lane l < kChunkEntries: the chunk's list word l
         uint2 ent;
         real4 pj;
         real2 ljj;
@@ -220,6 +221,8 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
             q.l.x = q.l.y = 0;
         }
         // the first entry and its j data (none for a chunk without entries)
+        q.words = make_uint2(0u, kNoMask);
+        if (lane < q.ch.w - q.ch.z) q.words = list[q.ch.z + lane];
         q.ent = make_uint2(0u, kNoMask);
         q.pj.x = q.pj.y = q.pj.z = q.pj.w = 0;
         q.ljj.x = q.ljj.y = 0;
@@ -258,6 +261,7 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
             p.w *= ke;
             ipos[lane] = p;
             ilj[lane] = cur.l;
+            if (lane < kChunkEntries) jw[lane] = cur.words;
         }
         wave_lds_sync();
         real fxi[8], fyi[8], fzi[8];
@@ -265,15 +269,22 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
         for (int k = 0; k < 8; ++k) fxi[k] = fyi[k] = fzi[k] = 0;
         double ec_acc = 0.0, el_acc = 0.0;
 
-        // software pipeline: entry e+1 (list word and j data) loads while entry e computes
-        uint2 ent = cur.ent;
+        // software pipeline: the j data of entry e+1 loads while entry e computes.
+        // The chunk's list words sit in LDS (written above with the i data); the
+        // word of entry e+1 is read at the end of entry e-1.  A scalar load of it
+        // would be simpler, but scalar loads return out of order, so every wait
+        // for an LDS read issued after one would also wait for it.
+        unsigned ex_n = __builtin_amdgcn_readfirstlane(cur.ent.x), ey_n = __builtin_amdgcn_readfirstlane(cur.ent.y);
+        uint2 wv = jw[1];
         real4 pj = cur.pj;
         real2 lj_j = cur.ljj;
         for (int e = e0; e < e1; ++e) {
-            // every lane loaded the same list word: move it to scalar registers so the
+            // the list word is the same on every lane: in scalar registers, the
             // per-cluster tests below are scalar branches
-            const unsigned ex = __builtin_amdgcn_readfirstlane(ent.x);
-            const unsigned ey = __builtin_amdgcn_readfirstlane(ent.y);
+            const unsigned ex = ex_n;
+            const unsigned ey = ey_n;
+            ex_n = __builtin_amdgcn_readfirstlane(wv.x);
+            ey_n = __builtin_amdgcn_readfirstlane(wv.y);
             const int cj = (int)(ex & 0xffffffu);
             const unsigned im = ex >> 24;
             const unsigned eid = ey & kNoMask, img = ey >> kImgShift;
@@ -303,12 +314,10 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
                 pjs.z = pj.z + sz;
                 pjs.w = pj.w;
             }
-            uint2 ent_n = ent;
             real4 pj_n = pj;
             real2 lj_n = lj_j;
             if (e + 1 < e1) {
-                ent_n = list[e + 1];
-                const int sjn = (int)(ent_n.x & 0xffffffu) * 8 + ja;
+                const int sjn = (int)(ex_n & 0xffffffu) * 8 + ja;
                 pj_n = pos[sjn];
                 lj_n = lj[sjn];
             }
@@ -318,26 +327,20 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
             // overlaps the last entries' pairs
             if (e == max(e0, e1 - 2)) tk = claim();
             real fxj = 0, fyj = 0, fzj = 0;
-            // i data of the next cluster is read from LDS one tile ahead of its use,
-            // so the read's latency overlaps the current tile's arithmetic.  The
-            // empty asm makes the LDS index opaque per entry: otherwise the compiler
-            // hoists all 8 clusters' reads out of the entry loop and holds them in 48
-            // more registers, which spills.
+            // i data of each cluster is read from LDS at its tile (reading it one
+            // tile ahead cost 6 registers and measured slower; the other waves on
+            // the SIMD hide the read).  The empty asm makes the LDS index opaque per
+            // entry: otherwise the compiler hoists all 8 clusters' reads out of the
+            // entry loop and holds them in 48 more registers, which spills.
             int io = ia;
             asm volatile("" : "+v"(io));
             const real4* ipe = ipos + io;
             const real2* ile = ilj + io;
-            real4 pi_n = ipe[0];
-            real2 li_n = ile[0];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const real4 pi = pi_n;
-                const real2 li = li_n;
-                if (k + 1 < 8) {
-                    pi_n = ipe[(k + 1) * 8];
-                    li_n = ile[(k + 1) * 8];
-                }
                 if (im & (1u << k)) {
+                    const real4 pi = ipe[k * 8];
+                    const real2 li = ile[k * 8];
                     // excluded lanes, as a wave-wide bit mask: listed exclusions, and on
                     // the diagonal tile the pairs with ja <= ia (self pairs and the other half)
                     const unsigned long long xm = em[k] | (cj == fc + k ? kLowerTriangle : 0ull);
@@ -401,8 +404,7 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
                 jf[(el * 3 + 1) * 8 + ja] = fyj;
                 jf[(el * 3 + 2) * 8 + ja] = fzj;
             }
-            if (lane == 0) jc[el] = cj;
-            ent = ent_n;
+            if (e + 2 < e1) wv = jw[el + 2];
             pj = pj_n;
             lj_j = lj_n;
         }
@@ -417,7 +419,7 @@ __device__ __forceinline__ void pair_body(const SysDev* __restrict__ sd, const i
             for (int b = 0; b < n; b += 8) {
                 const int el = b + ia;
                 if (el < n) {
-                    const int sj = jc[el] * 8 + ja;
+                    const int sj = (int)(jw[el].x & 0xffffffu) * 8 + ja;
                     facc_add(&FX[sj], jf[(el * 3 + 0) * 8 + ja]);
                     facc_add(&FY[sj], jf[(el * 3 + 1) * 8 + ja]);
                     facc_add(&FZ[sj], jf[(el * 3 + 2) * 8 + ja]);
