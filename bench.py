@@ -133,6 +133,12 @@ def time_path(sysm, prec, steps, warmup, skin):
     ctx.step(PAIR_SAMPLE_STEPS, sysm.dt)
     beside_ms, beside_n = ctx.kernel_time("pair")
     step_ev_ms, step_ev_n = ctx.kernel_time("step")
+    segs = {}
+    for name in ("side", "list", "update"):
+        try:
+            segs[name] = ctx.kernel_time(name)[0]
+        except Exception:          # not timed (direct launches, or no second stream)
+            segs[name] = None
     pairs1 = ctx.pair_count()
     # every kernel family alone, one stream, events around each
     ctx.enable_timing(L.TIMING_KERNELS)
@@ -145,8 +151,22 @@ def time_path(sysm, prec, steps, warmup, skin):
     pair = {"beside_pme_ms_per_launch": beside_ms, "beside_pme_launches": beside_n,
             "alone_ms_per_launch": kern["pair"]["ms_per_launch"], "alone_launches": kern["pair"]["launches"],
             "sampled_step_ms": step_ev_ms if step_ev_n else None,
+            # the same sampled steps: the terms and PME on the second stream (the
+            # chain beside the pair kernel), the list graph before the fork and
+            # the update after the join
+            "side_chain_ms_per_step": segs["side"], "list_ms_per_step": segs["list"],
+            "update_ms_per_step": segs["update"],
             "launch": "step graphs" if graphs else "direct launches (NEXUSMD_GRAPHS=0)"}
     return ctx, wall, kern, stats, 0.5 * (pairs0 + pairs1), pair
+
+
+def chains(pair):
+    """The two chains of a sampled step, in microseconds: the pair kernel on the
+    main stream and the terms and PME on the second stream beside it."""
+    us = lambda v: None if v is None else round(1e3 * v, 1)
+    return {"pair_beside_pme_us": us(pair["beside_pme_ms_per_launch"]),
+            "side_chain_us": us(pair["side_chain_ms_per_step"]), "list_us": us(pair["list_ms_per_step"]),
+            "update_us": us(pair["update_ms_per_step"])}
 
 
 def autotune_leg(sysm, prec, skin, steps_per_trial=TUNE_STEPS):
@@ -639,7 +659,7 @@ def main():
     ctx_m, sec_m, kern_m, stats_m, pairs, pair_m = time_path(sysm, L.MIXED, args.steps, args.warmup, args.skin)
     ctx_m.close()
     nsd_m = ns_per_day(args.steps, sysm.dt, sec_m)
-    log("mixed: %.3f ms/step, %.2f ns/day" % (1e3 * sec_m / args.steps, nsd_m))
+    log("mixed: %.3f ms/step, %.2f ns/day" % (1e3 * sec_m / args.steps, nsd_m), chains(pair_m))
     fp64 = None
     if not args.no_fp64:
         ctx_d, sec_d, kern_d, stats_d, _, pair_d = time_path(sysm, L.FP64, args.steps, args.warmup, args.skin)
@@ -647,7 +667,7 @@ def main():
         nsd_d = ns_per_day(args.steps, sysm.dt, sec_d)
         fp64 = {"ns_per_day": nsd_d, "ms_per_step": 1e3 * sec_d / args.steps, "steps": args.steps,
                 "pair": pair_d, "kernels": kern_d}
-        log("fp64: %.3f ms/step, %.2f ns/day" % (1e3 * sec_d / args.steps, nsd_d))
+        log("fp64: %.3f ms/step, %.2f ns/day" % (1e3 * sec_d / args.steps, nsd_d), chains(pair_d))
     steady = {"mixed": steady_state(sysm, L.MIXED, args.skin)}
     if fp64:
         steady["fp64"] = steady_state(sysm, L.FP64, args.skin)

@@ -178,7 +178,11 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
  *                         kernel (list before it, terms and PME beside it,
  *                         update after it) with events on the main stream
  *                         around the pair kernel and after every step: "pair"
- *                         (the pair kernel as it runs beside PME) and "step"
+ *                         (the pair kernel as it runs beside PME) and "step";
+ *                         around the graphs before the fork and after the
+ *                         join: "list" (the list kernels) and "update"; and
+ *                         on the second stream around the side graph: "side"
+ *                         (terms and PME as they run beside the pair kernel)
  * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
  * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);
  * NEXUSMD_OVERLAP=0 keeps every kernel on one stream.

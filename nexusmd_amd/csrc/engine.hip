@@ -934,11 +934,18 @@ class Engine : public EngineBase {
             hipEvent_t e0 = ev();
             HIPCHK(hipEventRecord(e0, B.stream));
             for (int i = 0; i < nsteps; ++i) {
-                hipEvent_t a = ev(), b = ev(), c = ev();
+                hipEvent_t a = ev(), b = ev(), c = ev(), la = ev(), lb = ev(), ua = ev();
+                HIPCHK(hipEventRecord(la, B.stream));
                 HIPCHK(hipGraphLaunch(seg_exec_[SEG_FIRST], B.stream));
+                HIPCHK(hipEventRecord(lb, B.stream));
+                evs_.push_back({"list", la, lb});
                 if (side) {
                     fork_side();
+                    hipEvent_t sa = ev(), sb = ev();
+                    HIPCHK(hipEventRecord(sa, side_));
                     HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], side_));
+                    HIPCHK(hipEventRecord(sb, side_));
+                    evs_.push_back({"side", sa, sb});
                 } else {
                     HIPCHK(hipGraphLaunch(seg_exec_[SEG_SIDE], B.stream));
                 }
@@ -947,8 +954,10 @@ class Engine : public EngineBase {
                 if (seg_exec_[SEG_PAIR]) HIPCHK(hipGraphLaunch(seg_exec_[SEG_PAIR], B.stream));
                 HIPCHK(hipEventRecord(b, B.stream));
                 if (side) join_side();
+                HIPCHK(hipEventRecord(ua, B.stream));
                 HIPCHK(hipGraphLaunch(seg_exec_[SEG_LAST], B.stream));
                 HIPCHK(hipEventRecord(c, B.stream));
+                evs_.push_back({"update", ua, c});
                 evs_.push_back({"pair", a, b});
                 evs_.push_back({"step", e0, c});
                 e0 = c;
