@@ -336,7 +336,7 @@ class Engine : public EngineBase {
                 tw_off += g[k];
                 fft_maxK_ = std::max(fft_maxK_, g[k]);
             }
-            // spread tiling: column blocks b x b and nz slabs whose LDS region fits 7680 points
+            // spread tiling: column blocks b x b and nz slabs whose LDS region fits kSpreadLdsPoints
             {
                 d.sp_margin = 0.5 * skin + 0.002;
                 long long best = -1;
@@ -356,8 +356,8 @@ class Engine : public EngineBase {
                         const int nx = ext(b * d.colw[0], g[0], d.L[0], b >= ncx);
                         const int ny = ext(b * d.colw[1], g[1], d.L[1], b >= ncy);
                         const int nzp = ext(d.L[2] / nz, g[2], d.L[2], nz == 1);
-                        const long long pts = (long long)nx * ny * nzp;
-                        if (pts > 7680) continue;
+                        const long long pts = (long long)nx * spread_plane(ny, nzp);
+                        if (pts > kSpreadLdsPoints) continue;
                         const long long tiles = (long long)((ncx + b - 1) / b) * ((ncy + b - 1) / b) * nz;
                         const long long cost = pts * tiles + 2000 * (tiles < 128 ? 128 - tiles : 0);
                         if (best < 0 || cost < best) { best = cost; bb = b; bnz = nz; }

@@ -24,6 +24,13 @@ constexpr int kPlaneThreads = 1024;
 constexpr int kXPassThreads = 512;
 constexpr size_t kMaxBlockLds = 160 * 1024 - 1024;   // less the kernels' static LDS
 
+// LDS-tiled spreading: the tile region holds at most kSpreadLdsPoints grid
+// points; its x planes are spread_plane(ny, nz) points apart, an odd count, so
+// that the five lanes of one atom (one x plane each) fall on five different
+// LDS banks (with an even stride they could all land on one).
+constexpr int kSpreadLdsPoints = 7680;   // 60 KB of int64 / double
+__host__ __device__ constexpr int spread_plane(int ny, int nz) { return (ny * nz) | 1; }
+
 // LDS bytes of a plane pass (two buffers, the y and z twiddles).
 template <typename R2>
 inline size_t plane_lds_bytes(int Ky, int Kz) {

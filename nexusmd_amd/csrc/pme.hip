@@ -126,6 +126,12 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
     }
 }
 
+// Quotient of non-negative integers n, d with n + d < 2^22 through a float
+// reciprocal: (n + 0.5) / d is at least 0.5 / d away from an integer, and the
+// two roundings (of 1/d and of the product) move it by at most
+// (n / d + 1) 2^-23, which is less.
+__device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
+
 // LDS-tiled spreading.  One workgroup per (system, column block, z slab):
 // the atoms binned into those columns at the last neighbour-list build whose
 // build-time z lies in the slab.  Their stencils are accumulated in an LDS copy
@@ -133,7 +139,6 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
 // atom may move between rebuilds, + 4 points for the order-5 stencil), then
 // the region is added to the global grid once, z-contiguous.  An atom outside
 // the region (never, while the rebuild criterion holds) adds directly.
-constexpr int kSpreadLdsPoints = 7680;   // 60 KB of int64 / double
 constexpr int kSpreadTileCols = 144;    // columns per block: at most 12 x 12 (the host's choice)
 
 template <class P>
@@ -181,7 +186,8 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
         }
     }
     const bool fx = (n[0] == Kx && lo[0] == 0), fy = (n[1] == Ky && lo[1] == 0), fz = (n[2] == Kz && lo[2] == 0);
-    const int npts = n[0] * n[1] * n[2];
+    const int pl = spread_plane(n[1], n[2]), nyz = n[1] * n[2];     // x-plane stride in LDS, points per plane
+    const int npts = n[0] * pl;
     if (npts > kSpreadLdsPoints) return;       // host guarantees the fit
     for (int i = threadIdx.x; i < npts; i += blockDim.x) g[i] = acc(0);
     __syncthreads();
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
                 for (int iz = 0; iz < 5; ++iz) {
                     int z = u2 + iz;
                     if (fz && z >= Kz) z -= Kz;
-                    grid_add_lds(&g[(x * n[1] + y) * n[2] + z], qxy * tz[iz]);
+                    grid_add_lds(&g[x * pl + y * n[2] + z], qxy * tz[iz]);
                 }
             }
         } else {
@@ -282,10 +288,13 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
     }
     __syncthreads();
     // flush the region (z fastest: contiguous global addresses)
+    const float inv_pl = 1.0f / (float)pl, inv_nz = 1.0f / (float)n[2];
     for (int i = threadIdx.x; i < npts; i += blockDim.x) {
         const acc v = g[i];
-        if (v == acc(0)) continue;
-        const int z = i % n[2], y = (i / n[2]) % n[1], x = i / (n[1] * n[2]);
+        if (v == acc(0)) continue;             // also the plane padding, never written
+        const int x = qdiv(i, inv_pl), r = i - x * pl;     // i, pl <= kSpreadLdsPoints
+        if (r >= nyz) continue;
+        const int y = qdiv(r, inv_nz), z = r - y * n[2];
         int gx = lo[0] + x, gy = lo[1] + y, gz = lo[2] + z;
         gx = gx < 0 ? gx + Kx : (gx >= Kx ? gx - Kx : gx);
         gy = gy < 0 ? gy + Ky : (gy >= Ky ? gy - Ky : gy);
@@ -442,10 +451,6 @@ __device__ __forceinline__ int next_radix(int m) {
     return 5;
 }
 
-// Quotient of small non-negative integers (n < 2^20, d < 2^12) through a float
-// reciprocal: (n + 0.5) / d stays at least 0.5 / d away from an integer, far
-// beyond the rounding of the product.
-__device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
 
 // Copies the N twiddles of an axis into LDS (all threads of the block).
 template <typename R2>
