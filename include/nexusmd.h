@@ -213,11 +213,15 @@ nx_status nx_get_tuning(nx_ctx* ctx, nx_tuning* t);
 nx_status nx_autotune(nx_ctx* ctx, int steps_per_trial, nx_tuning* chosen, double* ms_default, double* ms_best);
 
 /* Counters of the last call: pairs evaluated by the pair kernel (cluster
- * tiles x 64), pair-list entries, neighbour-list rebuilds. */
+ * tiles x 64), pair-list entries, neighbour-list rebuilds.  Over the
+ * context's life: steps run as the step graph without the list kernels
+ * (one-system contexts; NEXUSMD_LITE=0 turns it off), and steps run again
+ * because a step without the list kernels found a build due and froze. */
 typedef struct {
     int64_t n_slots, n_clusters, n_super, list_entries, tiles, rebuilds, steps;
     int grid[3];
     double alpha;
+    int64_t steps_without_list, steps_repeated;
 } nx_stats;
 nx_status nx_get_stats(nx_ctx* ctx, nx_stats* s);
 

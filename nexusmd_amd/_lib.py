@@ -60,7 +60,8 @@ class Energies(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("n_slots", C.c_int64), ("n_clusters", C.c_int64), ("n_super", C.c_int64),
                 ("list_entries", C.c_int64), ("tiles", C.c_int64), ("rebuilds", C.c_int64), ("steps", C.c_int64),
-                ("grid", C.c_int * 3), ("alpha", C.c_double)]
+                ("grid", C.c_int * 3), ("alpha", C.c_double), ("steps_without_list", C.c_int64),
+                ("steps_repeated", C.c_int64)]
 
 
 class Tuning(C.Structure):

@@ -217,6 +217,12 @@ class Engine : public EngineBase {
         HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+        // NEXUSMD_LITE=0: every step graph carries the list kernels; =2: the step
+        // without them is chosen without the forecast (every build then goes
+        // through a frozen step and its repeat; for tests of that path)
+        const char* li = std::getenv("NEXUSMD_LITE");
+        if (li && std::string(li) == "0") lite_mode_ = 0;
+        if (li && std::string(li) == "2") lite_mode_ = 2;
     }
     ~Engine() override {
         hipSetDevice(device_);
@@ -225,6 +231,7 @@ class Engine : public EngineBase {
         if (side_) hipStreamDestroy(side_);
         if (fork_) hipEventDestroy(fork_);
         if (join_) hipEventDestroy(join_);
+        if (info_host_) hipHostFree(info_host_);
         for (auto& e : evpool_) hipEventDestroy(e);
         for (auto& e : step_ev_) hipEventDestroy(e);
         if (copy_) hipStreamDestroy(copy_);
@@ -233,8 +240,7 @@ class Engine : public EngineBase {
 
     void release() {
         free_reports();
-        if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
-        if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+        destroy_graphs();
         destroy_segments();
         for (void* p : owned_) hipFree(p);
         owned_.clear();
@@ -447,6 +453,20 @@ class Engine : public EngineBase {
         B.pair_waves = B.pair_blocks * kPairWaves;
         dalloc(&B.wave_start, (size_t)B.pair_waves + 1, o, B.stream);
         dalloc(&B.pair_pool, (size_t)kPoolInts, o, B.stream);
+        if (B.nsys == 1 && lite_mode_) {
+            // the update's report to the host, for the choice of step graph
+            if (!info_host_) {
+                HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&info_host_), sizeof(StepInfo),
+                                     hipHostMallocMapped | hipHostMallocCoherent));
+            }
+            std::memset(info_host_, 0, sizeof(StepInfo));
+            HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&B.step_info), info_host_, 0));
+            dalloc(&B.step_seq, 1, o, B.stream);
+            dalloc(&B.step_max, (size_t)(B.n_atoms + 255) / 256, o, B.stream);
+            grow_n_ = 0;
+            last_seq_ = 0;
+        }
+        force_full_ = true;
         dalloc(&B.list, (size_t)B.n_list_cap, o, B.stream);
         dalloc(&B.emask, (size_t)B.n_emask_cap * 8, o, B.stream);
         dalloc(&B.rgrid, (size_t)B.rgrid_total, o, B.stream);
@@ -742,6 +762,7 @@ class Engine : public EngineBase {
         std::vector<SysCount> c(B.nsys);
         copy_sync(c.data(), B.d_cnt, sizeof(SysCount) * B.nsys, hipMemcpyDeviceToHost);
         for (auto& v : c) { v.rebuild = 1; v.overflow = 0; }
+        force_full_ = true;
         copy_sync(B.d_cnt, c.data(), sizeof(SysCount) * B.nsys, hipMemcpyHostToDevice);
     }
 
@@ -906,7 +927,7 @@ class Engine : public EngineBase {
         counts_ = c;
         failed_sys_.clear();
         for (int s = 0; s < B.nsys; ++s)
-            if (c[s].overflow) failed_sys_.push_back(s);
+            if (c[s].overflow & ~kStaleList) failed_sys_.push_back(s);
         if (!failed_sys_.empty()) {
             const int s = failed_sys_[0];
             throw HipError(NX_ERR_CAPACITY, "neighbour-list capacity exceeded in system " + std::to_string(s) +
@@ -967,6 +988,9 @@ class Engine : public EngineBase {
             // per-kernel HIP events need plain launches (events recorded in a graph
             // cannot be queried for elapsed time); the host enqueues ahead of the GPU
             for (int i = 0; i < nsteps; ++i) step_plain(dt, mask);
+        } else if (nsteps >= 4 && graphs_enabled_ && lite_ok(mask)) {
+            if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
+            run_lite(nsteps);
         } else if (nsteps >= 4 && graphs_enabled_) {
             if (!graph_exec_ || graph_dt_ != dt || graph_mask_ != mask) build_graph(dt, mask);
             for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(graph_exec_, B.stream));
@@ -1015,18 +1039,145 @@ class Engine : public EngineBase {
     }
 
     void build_graph(double dt, int mask) {
-        if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
-        if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+        destroy_graphs();
         capturing_ = true;
         HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
         forces(mask, false);
         launch_integrate<P>(B, dt);
         hipError_t e = hipStreamEndCapture(B.stream, &graph_);
+        if (e == hipSuccess && lite_ok(mask)) {
+            // the same step without the list kernels before the fork
+            e = hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                forces_lite(mask);
+                launch_integrate<P>(B, dt);
+                e = hipStreamEndCapture(B.stream, &graph_lite_);
+            }
+        }
         capturing_ = false;
         HIPCHK(e);
         HIPCHK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        if (graph_lite_) HIPCHK(hipGraphInstantiate(&graph_lite_exec_, graph_lite_, nullptr, nullptr, 0));
         graph_dt_ = dt;
         graph_mask_ = mask;
+    }
+    void destroy_graphs() {
+        if (graph_exec_) { hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+        if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+        if (graph_lite_exec_) { hipGraphExecDestroy(graph_lite_exec_); graph_lite_exec_ = nullptr; }
+        if (graph_lite_) { hipGraphDestroy(graph_lite_); graph_lite_ = nullptr; }
+    }
+
+    // ---------------------------------------------- steps without the list kernels
+    //
+    // On most steps no list is built, and the five list kernels before the fork
+    // only read the rebuild flag and exit; their launches still cost the step
+    // about 15 us on the 100k-atom system.  A one-system batch therefore has a
+    // second step graph without them (nl_waves still runs, after the fork).
+    // The host picks one of the two before each step, from the report nl_waves
+    // writes to host memory early in every step (StepInfo): the largest
+    // displacement from the build positions after the update before it, and
+    // its growth per step.  The
+    // device keeps the result exact whatever the pick: a step without the list
+    // kernels on a system whose rebuild flag is up marks it stale (nl_waves),
+    // the update freezes it, and the host runs that step again with the list
+    // kernels.  Every build therefore happens at the step its flag names, as
+    // with the list kernels in every step, and the results are the same bits.
+    bool lite_ok(int mask) const {
+        return lite_mode_ && B.step_info && B.nsys == 1 && use_side(mask) && B.timing == NX_TIMING_OFF;
+    }
+    void forces_lite(int mask) {
+        const bool nb = (mask & NX_FORCE_NONBONDED) != 0;
+        fork_side();
+        on_side([&] { side_work(mask, false); });
+        launch_nlist<P>(B, !nb, 3);
+        launch_pair_kernel<P>(B, false);
+        join_side();
+    }
+    // the growth of the largest displacement per step, from consecutive reports
+    void observe(const volatile StepInfo* in) {
+        const unsigned long long seq = in->seq;
+        const float d = in->dmax;
+        if (seq == last_seq_ + 1 && d >= last_dmax_ && last_seq_ > 0) {
+            grow_[grow_n_ % kGrowWindow] = d - last_dmax_;
+            ++grow_n_;
+        }
+        if (seq != last_seq_) { last_seq_ = seq; last_dmax_ = d; }
+    }
+    void run_lite(int nsteps) {
+        const volatile StepInfo* in = info_host_;
+        const double half = std::sqrt(B.sys[0].skin_half2);
+        constexpr int R = 4;        // steps whose graph choice is remembered
+        std::vector<SysCount> c(1);
+        HIPCHK(hipStreamSynchronize(B.stream));
+        copy_sync(c.data(), B.d_cnt, sizeof(SysCount), hipMemcpyDeviceToHost);
+        const int done0 = c[0].steps_done;
+        long long remaining = nsteps, launched_total = 0;
+        // Rounds: launch the remaining steps, choosing each graph from the report
+        // of the step before it; then read the exact count of steps done (a step
+        // without the list kernels that found a build due froze and is run again
+        // in the next round).
+        while (remaining > 0 && !(c[0].overflow & ~kStaleList)) {
+            const unsigned long long seq0 = in->seq;       // every earlier step has reported
+            bool full_of[R] = {};
+            for (long long j = 0; j < remaining; ++j) {
+                bool full;
+                if (j == 0) {
+                    full = force_full_ || c[0].rebuild || (c[0].overflow & kStaleList);
+                } else {
+                    // Step j is chosen from the report of step j - 1, which its nl_waves
+                    // writes early in that step: graph j is then queued while step j - 1
+                    // runs.  (Waiting for the report of step j - 2 instead forecast two
+                    // steps ahead and ran fewer steps without the list kernels: 0.2253
+                    // against 0.2227 ms per 100k-atom mixed step.)
+                    if ((long long)(in->seq - seq0) < j) {
+                        // host memory only (a stream query in the loop delayed the GPU's
+                        // work by several us a step); after 0.2 s without the report the
+                        // stream is asked, and a drained stream without it is an error
+                        const auto t0 = std::chrono::steady_clock::now();
+                        for (unsigned spin = 1; (long long)(in->seq - seq0) < j; ++spin) {
+                            if ((spin & 4095u) != 0) continue;
+                            if (std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(200)) continue;
+                            const hipError_t q = hipStreamQuery(B.stream);
+                            if (q == hipSuccess && (long long)(in->seq - seq0) < j)
+                                throw HipError(NX_ERR_STATE, "a finished step left no report in host memory");
+                            if (q != hipSuccess && q != hipErrorNotReady) HIPCHK(q);
+                        }
+                    }
+                    observe(in);
+                    const bool due = in->rebuild != 0, prev_full = full_of[(j - 1) % R];
+                    if (due && !prev_full) {
+                        full = true;             // step j - 1 freezes; step j builds
+                    } else if (lite_mode_ == 2) {
+                        full = false;
+                    } else {
+                        // step j reads the positions after step j - 1's update: the
+                        // displacement of the report (after the update before step j - 1;
+                        // zero if step j - 1 builds) plus one step's growth, taken as
+                        // twice the largest of the recent steps, plus a margin
+                        float g = 0.0f;
+                        const int n = std::min(grow_n_, kGrowWindow);
+                        for (int i = 0; i < n; ++i) g = std::max(g, grow_[i]);
+                        const double base = due ? 0.0 : (double)in->dmax;
+                        full = n < 2 || base + 2.0 * g + 0.02 * half >= half;
+                    }
+                }
+                HIPCHK(hipGraphLaunch(full ? graph_exec_ : graph_lite_exec_, B.stream));
+                full_of[j % R] = full;
+                if (full) force_full_ = false;
+                if (full) ++lite_full_; else ++lite_lite_;
+                ++launched_total;
+            }
+            HIPCHK(hipStreamSynchronize(B.stream));
+            copy_sync(c.data(), B.d_cnt, sizeof(SysCount), hipMemcpyDeviceToHost);
+            const long long left = nsteps - (c[0].steps_done - done0);
+            if (left >= remaining && !(c[0].overflow & ~kStaleList))
+                throw HipError(NX_ERR_STATE, "a round of steps made no progress");
+            remaining = left;
+        }
+        const long long executed = c[0].steps_done - done0;
+        lite_repeats_ += launched_total - executed;
+        steps_ += executed;
     }
 
     void build_segments(double dt, int mask) {
@@ -1321,6 +1472,8 @@ class Engine : public EngineBase {
         st->steps = steps_;
         for (int k = 0; k < 3; ++k) st->grid[k] = B.sys.empty() ? 0 : B.sys[0].grid[k];
         st->alpha = B.sys.empty() ? 0 : B.sys[0].alpha;
+        st->steps_without_list = lite_lite_;
+        st->steps_repeated = lite_repeats_;
     }
 
     bool kernel_time(const std::string& name, double* ms, long long* n) override {
@@ -1499,6 +1652,17 @@ class Engine : public EngineBase {
     bool graphs_enabled_ = true;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_lite_ = nullptr;       // the step without the list kernels before the fork
+    hipGraphExec_t graph_lite_exec_ = nullptr;
+    int lite_mode_ = 1;                     // 0 off, 1 forecast, 2 no forecast (tests)
+    StepInfo* info_host_ = nullptr;         // pinned, mapped: the update's report
+    static constexpr int kGrowWindow = 16;
+    float grow_[kGrowWindow] = {};
+    int grow_n_ = 0;
+    unsigned long long last_seq_ = 0;
+    float last_dmax_ = 0.0f;
+    bool force_full_ = true;                // the next step must carry the list kernels
+    long long lite_full_ = 0, lite_lite_ = 0, lite_repeats_ = 0;   // step graphs launched, steps run again
     double graph_dt_ = 0.0;
     int graph_mask_ = -1;
     std::vector<int> failed_sys_;           // systems whose list overflowed in the last call

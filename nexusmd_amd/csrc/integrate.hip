@@ -98,6 +98,23 @@ __global__ void kinetic_kernel(const int* __restrict__ atom_sys, const int* __re
     wave_energy_add(energy, s < 0 ? 0 : s, E_KIN, ke, s >= 0);
 }
 
+// The largest squared displacement of the block's atoms, into blockmax
+// (one plain store per block; every thread calls it).  The next nl_waves
+// launch reduces the blocks' values and reports the maximum to the host.
+__device__ __forceinline__ void block_max_displacement(float* blockmax, float dm2) {
+    __shared__ float wmax[16];
+    float m = dm2;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) wmax[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nw; ++w) m = fmaxf(m, wmax[w]);
+        blockmax[blockIdx.x] = m;
+    }
+}
+
 // Leapfrog update; also writes the slot-order position the next step's force
 // kernels read (x, y, z, q at the atom's slot) and clears the slot force it
 // has consumed, so no packing pass runs between builds.
@@ -109,11 +126,12 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
                                  const double* __restrict__ YB, const double* __restrict__ ZB,
                                  typename P::facc* FAx, typename P::facc* FAy, typename P::facc* FAz,
                                  typename P::facc* FSx, typename P::facc* FSy, typename P::facc* FSz,
-                                 typename P::real4* pos, double dt, int n_atoms) {
+                                 typename P::real4* pos, double dt, int n_atoms, float* blockmax) {
     using real = typename P::real;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     int s = 0;
+    float dm2 = 0.0f;       // squared displacement from the build position
     if (a < n_atoms) {
         s = atom_sys[a];
         const SysDev& d = sd[s];
@@ -152,9 +170,11 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
             pos[sl] = slot_position<P>(x, y, z, Q[a]);
         const double d2 = dx * dx + dy * dy + dz * dz;
         moved = !(d2 <= d.skin_half2);
+        dm2 = (float)d2;
     }
     // lanes of a wave storing to one flag merge into one write
     if (moved) sc[s].rebuild = 1;
+    if (blockmax) block_max_displacement(blockmax, dm2);
 }
 
 template <class P>
@@ -227,7 +247,7 @@ void launch_integrate(Batch<P>& B, double dt) {
     const int T = 256;
     integrate_kernel<P><<<(unsigned)((B.n_atoms + T - 1) / T), T, 0, B.stream>>>(
         B.d_sys, B.d_cnt, B.atom_sys, B.slot_of_atom, B.X, B.Y, B.Z, B.VX, B.VY, B.VZ, B.invm, B.q, B.XB, B.YB, B.ZB,
-        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.pos, dt, B.n_atoms);
+        B.FAx, B.FAy, B.FAz, B.FSx, B.FSy, B.FSz, B.pos, dt, B.n_atoms, B.step_max);
 }
 
 template <class P>

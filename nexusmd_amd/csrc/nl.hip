@@ -104,6 +104,9 @@ __global__ void nl_scan(const SysDev* sd, SysCount* sc, int* col_tmp, int* col_c
     if (!sc[s].rebuild) return;
     const SysDev& d = sd[s];
     const int nc = d.ncx * d.ncy;
+    // the build now happens: a stale-list mark of an earlier step without the
+    // list kernels is cleared before the kernels after this one read the flags
+    if (threadIdx.x == 0 && (sc[s].overflow & kStaleList)) sc[s].overflow &= ~kStaleList;
     __shared__ int tot_slot, tot_sup, carry_slot, carry_sup;
     __shared__ int ws[1024], wsup[1024];
     if (threadIdx.x == 0) { carry_slot = 0; carry_sup = 0; }
@@ -682,26 +685,63 @@ constexpr int kWavesLdsSys = 1024;     // systems whose chunk prefix nl_waves ke
 
 // With `pool` non-null it also does what the pair kernel's set-up launch does
 // (after every read of the flags): clears the rebuild flags, counts the
-// builds and resets the pair kernel's claim counters.
-__device__ __forceinline__ void waves_pair_begin(SysCount* sc, int nsys, int* pool) {
+// builds and resets the pair kernel's claim counters.  In a step without the
+// list kernels (`lite`) a raised flag is kept and the system is marked as
+// having a stale list instead: the update then freezes it for the step.
+// With `info` non-null (a one-system batch) thread 0 also reports to host
+// memory what the host needs to choose the next step graphs: the largest
+// displacement after the last update (`dmax2`, reduced from the update's
+// per-block values), whether this step builds a list (or, without the list
+// kernels, froze), the steps done, and the report count.  The report is a
+// forecast aid only (the device freezes a step whose list is due whatever the
+// host chose), so its stores are not fenced against each other.
+__device__ __forceinline__ void waves_pair_begin(SysCount* sc, int nsys, int* pool, int lite,
+                                                 unsigned long long* step_seq, StepInfo* info, float dmax2) {
     const int tid = threadIdx.x;
     if (tid < 8) pool[tid * kPoolStride] = 0;
-    for (int s = tid; s < nsys; s += blockDim.x)
-        if (sc[s].rebuild) {
-            sc[s].rebuild = 0;
-            sc[s].rebuilds += 1;
+    for (int s = tid; s < nsys; s += blockDim.x) {
+        const int due = sc[s].rebuild;
+        if (due) {
+            if (lite) {
+                sc[s].overflow |= kStaleList;
+            } else {
+                sc[s].rebuild = 0;
+                sc[s].rebuilds += 1;
+            }
         }
+        if (info && s == 0) {
+            const unsigned long long n = *step_seq + 1;     // one thread of one block per launch
+            *step_seq = n;
+            volatile StepInfo* vi = info;
+            vi->dmax = sqrtf(dmax2);
+            vi->rebuild = due;
+            vi->overflow = sc[0].overflow;
+            vi->steps_done = sc[0].steps_done;
+            vi->seq = n;
+        }
+    }
 }
 
 __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost,
-                                                 int* prefix, int* wave_start, int NW, int* pool) {
+                                                 int* prefix, int* wave_start, int NW, int* pool, int lite,
+                                                 unsigned long long* step_seq, StepInfo* info,
+                                                 const float* step_max, int n_step_max) {
     __shared__ int any, total;
     __shared__ long long wsum[16];
+    __shared__ float dmax_s[16];
     __shared__ int pre_s[kWavesLdsSys + 1], off_s[kWavesLdsSys];
     const int tid = threadIdx.x;
+    if (info) {         // the largest of the update's per-block displacements
+        float m = 0.0f;
+        for (int i = tid; i < n_step_max; i += blockDim.x) m = fmaxf(m, step_max[i]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((tid & 63) == 0) dmax_s[tid >> 6] = m;
+    }
     if (tid == 0) {
         int a = 0;
-        for (int s = 0; s < nsys; ++s) a |= sc[s].rebuild;
+        if (!lite)      // no list was built: the ranges of the last build stand
+            for (int s = 0; s < nsys; ++s) a |= sc[s].rebuild;
         if (a) {
             int t = 0;
             for (int s = 0; s < nsys; ++s) {
@@ -718,7 +758,10 @@ __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc,
         any = a;
     }
     __syncthreads();
-    if (pool) waves_pair_begin(sc, nsys, pool);       // the flags were read above
+    float dmax2 = 0.0f;
+    if (info && tid == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) dmax2 = fmaxf(dmax2, dmax_s[w]);
+    if (pool) waves_pair_begin(sc, nsys, pool, lite, step_seq, info, dmax2);       // the flags were read above
     if (!any) return;
     const int T = total;
     const bool lds = nsys <= kWavesLdsSys;
@@ -797,9 +840,11 @@ void launch_nlist(Batch<P>& B, bool clear_flags, int part) {
     const int T = 256;
     hipStream_t st = B.stream;
     auto nb = [](long n, int t) { return (unsigned)((n + t - 1) / t); };
-    if (part == 2) {       // the last kernel alone, doing the pair kernel's set-up too
+    if (part == 2 || part == 3) {   // the last kernel alone, doing the pair kernel's set-up too
+        // (part 3: in a step without the other list kernels)
         nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves,
-                                     B.pair_pool);
+                                     B.pair_pool, part == 3 ? 1 : 0, B.step_seq, B.step_info, B.step_max,
+                                     B.step_info ? (B.n_atoms + 255) / 256 : 0);
         return;
     }
     nl_bin<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.atom_col, B.col_tmp,
@@ -816,7 +861,7 @@ void launch_nlist(Batch<P>& B, bool clear_flags, int part) {
                                               B.excl_idx, B.list, B.emask, B.chunks, B.chunk_cost, B.n_super_cap);
     if (part == 1) return;  // all but the last kernel
     nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves,
-                                 nullptr);
+                                 nullptr, 0, nullptr, nullptr, nullptr, 0);
     if (clear_flags) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
 }
 

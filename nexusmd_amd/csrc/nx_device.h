@@ -66,6 +66,21 @@ struct SysCount {
     int emask_count, rebuild, overflow, rebuilds;
     int n_chunks, steps_done, pad1, pad2;   // steps_done: leapfrog steps the system completed
 };
+// overflow bit of a system that ran a step without the list kernels although
+// its list was due for a build: the update froze it for that step (the host
+// runs the step again with the list kernels; not a failure)
+constexpr int kStaleList = 8;
+
+// What nl_waves of a one-system batch reports to host memory early in every
+// step, for the host's choice between the step graphs with and without the
+// list kernels: the report count, the largest displacement of an atom from
+// its build position after the last update, the steps done, whether this
+// step's list is due, and the overflow flags.
+struct StepInfo {
+    unsigned long long seq;     // reports so far
+    float dmax;                 // nm
+    int steps_done, rebuild, overflow;
+};
 
 constexpr int kChunkEntries = 32;       // pair-list entries per chunk, the pair kernel's unit of work
 // Pair-list entries one super-cluster may hold (the list build keeps them in

@@ -88,6 +88,12 @@ struct Batch {
     int* chunk_cost = nullptr;      // per chunk: estimated pair-kernel work (list build)
     int* wave_start = nullptr;      // [pair_waves + 1] chunk ranges of equal estimated work (per XCD: its waves' union)
     int* pair_pool = nullptr;       // [kPoolInts] per XCD: chunks claimed so far by the running pair kernel
+    // one-system batches: nl_waves' report to the host (host memory, mapped),
+    // the report count (device) and the update's largest squared displacement
+    // per 256-atom block (device)
+    StepInfo* step_info = nullptr;
+    unsigned long long* step_seq = nullptr;
+    float* step_max = nullptr;
     int pair_blocks = 0, pair_waves = 0;
     int n_cu = 0;                   // compute units of the device
     int n_chunk_cap = 0;
