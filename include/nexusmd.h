@@ -185,7 +185,8 @@ nx_status nx_neighbor_pairs(nx_ctx* ctx, int* pairs, int64_t cap, int64_t* count
  *                         (terms and PME as they run beside the pair kernel)
  * Environment, read by nx_create: NEXUSMD_GRAPHS=0 makes every step run as
  * direct launches (NX_TIMING_GRAPH_PAIR then times as NX_TIMING_KERNELS);
- * NEXUSMD_OVERLAP=0 keeps every kernel on one stream.
+ * NEXUSMD_OVERLAP=0 keeps every kernel on one stream; NEXUSMD_LITE=0 keeps
+ * the list kernels in every step graph of a one-system context.
  * No counterpart in the reference (it has no GPU path to time). */
 enum { NX_TIMING_OFF = 0, NX_TIMING_KERNELS = 1, NX_TIMING_GRAPH_PAIR = 2 };
 nx_status nx_enable_timing(nx_ctx* ctx, int mode);
