@@ -462,7 +462,7 @@ class Engine : public EngineBase {
             std::memset(info_host_, 0, sizeof(StepInfo));
             HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&B.step_info), info_host_, 0));
             dalloc(&B.step_seq, 1, o, B.stream);
-            dalloc(&B.step_max, (size_t)(B.n_atoms + 255) / 256, o, B.stream);
+            dalloc(&B.step_max, (size_t)kStepMaxSlots, o, B.stream);
             grow_n_ = 0;
             last_seq_ = 0;
         }
@@ -1096,7 +1096,7 @@ class Engine : public EngineBase {
     }
     // the growth of the largest displacement per step, from consecutive reports
     void observe(const volatile StepInfo* in) {
-        const unsigned long long seq = in->seq;
+        const unsigned seq = in->seq;
         const float d = in->dmax;
         if (seq == last_seq_ + 1 && d >= last_dmax_ && last_seq_ > 0) {
             grow_[grow_n_ % kGrowWindow] = d - last_dmax_;
@@ -1118,34 +1118,34 @@ class Engine : public EngineBase {
         // without the list kernels that found a build due froze and is run again
         // in the next round).
         while (remaining > 0 && !(c[0].overflow & ~kStaleList)) {
-            const unsigned long long seq0 = in->seq;       // every earlier step has reported
+            const unsigned seq0 = in->seq;       // every earlier step has reported
             bool full_of[R] = {};
             for (long long j = 0; j < remaining; ++j) {
                 bool full;
                 if (j == 0) {
-                    full = force_full_ || c[0].rebuild || (c[0].overflow & kStaleList);
+                    full = force_full_ || c[0].rebuild || (c[0].overflow & kStaleList);     // exact
                 } else {
                     // Step j is chosen from the report of step j - 1, which its nl_waves
                     // writes early in that step: graph j is then queued while step j - 1
                     // runs.  (Waiting for the report of step j - 2 instead forecast two
                     // steps ahead and ran fewer steps without the list kernels: 0.2253
                     // against 0.2227 ms per 100k-atom mixed step.)
-                    if ((long long)(in->seq - seq0) < j) {
+                    if ((long long)(unsigned)(in->seq - seq0) < j) {
                         // host memory only (a stream query in the loop delayed the GPU's
                         // work by several us a step); after 0.2 s without the report the
                         // stream is asked, and a drained stream without it is an error
                         const auto t0 = std::chrono::steady_clock::now();
-                        for (unsigned spin = 1; (long long)(in->seq - seq0) < j; ++spin) {
+                        for (unsigned spin = 1; (long long)(unsigned)(in->seq - seq0) < j; ++spin) {
                             if ((spin & 4095u) != 0) continue;
                             if (std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(200)) continue;
                             const hipError_t q = hipStreamQuery(B.stream);
-                            if (q == hipSuccess && (long long)(in->seq - seq0) < j)
+                            if (q == hipSuccess && (long long)(unsigned)(in->seq - seq0) < j)
                                 throw HipError(NX_ERR_STATE, "a finished step left no report in host memory");
                             if (q != hipSuccess && q != hipErrorNotReady) HIPCHK(q);
                         }
                     }
                     observe(in);
-                    const bool due = in->rebuild != 0, prev_full = full_of[(j - 1) % R];
+                    const bool due = (in->flags & 1) != 0, prev_full = full_of[(j - 1) % R];
                     if (due && !prev_full) {
                         full = true;             // step j - 1 freezes; step j builds
                     } else if (lite_mode_ == 2) {
@@ -1659,7 +1659,7 @@ class Engine : public EngineBase {
     static constexpr int kGrowWindow = 16;
     float grow_[kGrowWindow] = {};
     int grow_n_ = 0;
-    unsigned long long last_seq_ = 0;
+    unsigned last_seq_ = 0;
     float last_dmax_ = 0.0f;
     bool force_full_ = true;                // the next step must carry the list kernels
     long long lite_full_ = 0, lite_lite_ = 0, lite_repeats_ = 0;   // step graphs launched, steps run again

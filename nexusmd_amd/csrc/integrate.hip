@@ -98,10 +98,11 @@ __global__ void kinetic_kernel(const int* __restrict__ atom_sys, const int* __re
     wave_energy_add(energy, s < 0 ? 0 : s, E_KIN, ke, s >= 0);
 }
 
-// The largest squared displacement of the block's atoms, into blockmax
-// (one plain store per block; every thread calls it).  The next nl_waves
-// launch reduces the blocks' values and reports the maximum to the host.
-__device__ __forceinline__ void block_max_displacement(float* blockmax, float dm2) {
+// The largest squared displacement of the block's atoms, into slot
+// blockIdx % kStepMaxSlots of step_max (one atomic per block, few blocks per
+// slot; every thread calls it).  The next nl_waves launch reads and clears
+// the slots and reports the maximum to the host.
+__device__ __forceinline__ void block_max_displacement(unsigned* step_max, float dm2) {
     __shared__ float wmax[16];
     float m = dm2;
 #pragma unroll
@@ -111,7 +112,8 @@ __device__ __forceinline__ void block_max_displacement(float* blockmax, float dm
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < nw; ++w) m = fmaxf(m, wmax[w]);
-        blockmax[blockIdx.x] = m;
+        // non-negative floats order as their bit patterns
+        atomicMax(&step_max[blockIdx.x % kStepMaxSlots], __float_as_uint(m));
     }
 }
 
@@ -126,7 +128,7 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
                                  const double* __restrict__ YB, const double* __restrict__ ZB,
                                  typename P::facc* FAx, typename P::facc* FAy, typename P::facc* FAz,
                                  typename P::facc* FSx, typename P::facc* FSy, typename P::facc* FSz,
-                                 typename P::real4* pos, double dt, int n_atoms, float* blockmax) {
+                                 typename P::real4* pos, double dt, int n_atoms, unsigned* step_max) {
     using real = typename P::real;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
@@ -174,7 +176,7 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
     }
     // lanes of a wave storing to one flag merge into one write
     if (moved) sc[s].rebuild = 1;
-    if (blockmax) block_max_displacement(blockmax, dm2);
+    if (step_max) block_max_displacement(step_max, dm2);
 }
 
 template <class P>

@@ -690,13 +690,13 @@ constexpr int kWavesLdsSys = 1024;     // systems whose chunk prefix nl_waves ke
 // having a stale list instead: the update then freezes it for the step.
 // With `info` non-null (a one-system batch) thread 0 also reports to host
 // memory what the host needs to choose the next step graphs: the largest
-// displacement after the last update (`dmax2`, reduced from the update's
-// per-block values), whether this step builds a list (or, without the list
-// kernels, froze), the steps done, and the report count.  The report is a
-// forecast aid only (the device freezes a step whose list is due whatever the
-// host chose), so its stores are not fenced against each other.
-__device__ __forceinline__ void waves_pair_begin(SysCount* sc, int nsys, int* pool, int lite,
-                                                 unsigned long long* step_seq, StepInfo* info, float dmax2) {
+// displacement after the last update (`dmax2`, from the update's slots),
+// whether this step builds a list (or, without the list kernels, froze), the
+// steps done, and the report count, in one 16-byte store.  The report is a
+// forecast aid only: the device freezes a step whose list is due whatever
+// the host chose.
+__device__ __forceinline__ void waves_pair_begin(SysCount* sc, int nsys, int* pool, int lite, unsigned* step_seq,
+                                                 StepInfo* info, float dmax2) {
     const int tid = threadIdx.x;
     if (tid < 8) pool[tid * kPoolStride] = 0;
     for (int s = tid; s < nsys; s += blockDim.x) {
@@ -710,33 +710,32 @@ __device__ __forceinline__ void waves_pair_begin(SysCount* sc, int nsys, int* po
             }
         }
         if (info && s == 0) {
-            const unsigned long long n = *step_seq + 1;     // one thread of one block per launch
+            const unsigned n = *step_seq + 1u;     // one thread of one block per launch
             *step_seq = n;
-            volatile StepInfo* vi = info;
-            vi->dmax = sqrtf(dmax2);
-            vi->rebuild = due;
-            vi->overflow = sc[0].overflow;
-            vi->steps_done = sc[0].steps_done;
-            vi->seq = n;
+            const int flags = (due ? 1 : 0) | (sc[0].overflow << 1);
+            *reinterpret_cast<uint4*>(info) =
+                make_uint4(n, __float_as_uint(sqrtf(dmax2)), (unsigned)sc[0].steps_done, (unsigned)flags);
         }
     }
 }
 
 __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc, int nsys, const int* chunk_cost,
                                                  int* prefix, int* wave_start, int NW, int* pool, int lite,
-                                                 unsigned long long* step_seq, StepInfo* info,
-                                                 const float* step_max, int n_step_max) {
+                                                 unsigned* step_seq, StepInfo* info, unsigned* step_max) {
     __shared__ int any, total;
     __shared__ long long wsum[16];
-    __shared__ float dmax_s[16];
+    __shared__ float dmax_s[1];
     __shared__ int pre_s[kWavesLdsSys + 1], off_s[kWavesLdsSys];
     const int tid = threadIdx.x;
-    if (info) {         // the largest of the update's per-block displacements
+    if (info && tid < 64) {     // the largest of the update's slots (wave 0), which it clears
         float m = 0.0f;
-        for (int i = tid; i < n_step_max; i += blockDim.x) m = fmaxf(m, step_max[i]);
+        if (tid < kStepMaxSlots) {
+            m = __uint_as_float(step_max[tid]);
+            step_max[tid] = 0u;
+        }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if ((tid & 63) == 0) dmax_s[tid >> 6] = m;
+        if (tid == 0) dmax_s[0] = m;
     }
     if (tid == 0) {
         int a = 0;
@@ -758,9 +757,7 @@ __global__ __launch_bounds__(1024) void nl_waves(const SysDev* sd, SysCount* sc,
         any = a;
     }
     __syncthreads();
-    float dmax2 = 0.0f;
-    if (info && tid == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) dmax2 = fmaxf(dmax2, dmax_s[w]);
+    const float dmax2 = (info && tid == 0) ? dmax_s[0] : 0.0f;
     if (pool) waves_pair_begin(sc, nsys, pool, lite, step_seq, info, dmax2);       // the flags were read above
     if (!any) return;
     const int T = total;
@@ -843,8 +840,7 @@ void launch_nlist(Batch<P>& B, bool clear_flags, int part) {
     if (part == 2 || part == 3) {   // the last kernel alone, doing the pair kernel's set-up too
         // (part 3: in a step without the other list kernels)
         nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves,
-                                     B.pair_pool, part == 3 ? 1 : 0, B.step_seq, B.step_info, B.step_max,
-                                     B.step_info ? (B.n_atoms + 255) / 256 : 0);
+                                     B.pair_pool, part == 3 ? 1 : 0, B.step_seq, B.step_info, B.step_max);
         return;
     }
     nl_bin<<<nb(B.n_atoms, T), T, 0, st>>>(B.d_sys, B.d_cnt, B.atom_sys, B.X, B.Y, B.Z, B.atom_col, B.col_tmp,
@@ -861,7 +857,7 @@ void launch_nlist(Batch<P>& B, bool clear_flags, int part) {
                                               B.excl_idx, B.list, B.emask, B.chunks, B.chunk_cost, B.n_super_cap);
     if (part == 1) return;  // all but the last kernel
     nl_waves<<<1, 1024, 0, st>>>(B.d_sys, B.d_cnt, B.nsys, B.chunk_cost, B.chunk_prefix, B.wave_start, B.pair_waves,
-                                 nullptr, 0, nullptr, nullptr, nullptr, 0);
+                                 nullptr, 0, nullptr, nullptr, nullptr);
     if (clear_flags) nl_clear_flag<<<nb(B.nsys, 64), 64, 0, st>>>(B.d_cnt, B.nsys);
 }
 

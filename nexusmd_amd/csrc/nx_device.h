@@ -76,11 +76,14 @@ constexpr int kStaleList = 8;
 // list kernels: the report count, the largest displacement of an atom from
 // its build position after the last update, the steps done, whether this
 // step's list is due, and the overflow flags.
-struct StepInfo {
-    unsigned long long seq;     // reports so far
+struct __align__(16) StepInfo {
+    unsigned seq;               // reports so far (modulo 2^32)
     float dmax;                 // nm
-    int steps_done, rebuild, overflow;
+    int steps_done;
+    int flags;                  // bit 0: the list is due; bits 1..: the overflow flags
 };
+// slots of the update's largest squared displacement (block b adds to slot b % 64)
+constexpr int kStepMaxSlots = 64;
 
 constexpr int kChunkEntries = 32;       // pair-list entries per chunk, the pair kernel's unit of work
 // Pair-list entries one super-cluster may hold (the list build keeps them in

@@ -90,10 +90,10 @@ struct Batch {
     int* pair_pool = nullptr;       // [kPoolInts] per XCD: chunks claimed so far by the running pair kernel
     // one-system batches: nl_waves' report to the host (host memory, mapped),
     // the report count (device) and the update's largest squared displacement
-    // per 256-atom block (device)
+    // (device, kStepMaxSlots float bit patterns)
     StepInfo* step_info = nullptr;
-    unsigned long long* step_seq = nullptr;
-    float* step_max = nullptr;
+    unsigned* step_seq = nullptr;
+    unsigned* step_max = nullptr;
     int pair_blocks = 0, pair_waves = 0;
     int n_cu = 0;                   // compute units of the device
     int n_chunk_cap = 0;
