@@ -1612,7 +1612,7 @@ class Engine : public EngineBase {
     static size_t footprint_bytes(const Batch<P>& b) {
         size_t A = b.n_atoms, S = b.n_slots_cap;
         return A * (14 * 8 + 3 * 4 + 3 * sizeof(facc)) + S * (3 * 4 + sizeof(real4) + sizeof(real2) + 3 * sizeof(facc)) +
-               (size_t)b.n_list_cap * 8 + (size_t)b.n_emask_cap * 64 + (size_t)b.rgrid_total * (sizeof(facc) + sizeof(real)) +
+               (size_t)b.n_list_cap * 8 + (size_t)b.n_emask_cap * 64 + (size_t)b.rgrid_total * (sizeof(typename P::gacc) + sizeof(real)) +
                (size_t)b.cgrid_total * (sizeof(real2) + sizeof(real));
     }
 

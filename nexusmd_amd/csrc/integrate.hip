@@ -49,9 +49,9 @@ __global__ void pack_kernel(const SysDev* __restrict__ sd, const SysCount* __res
         p.w = 0;
     }
     pos[sl] = p;
-    FX[sl] = 0;
-    FY[sl] = 0;
-    FZ[sl] = 0;
+    FX[sl] = typename P::facc{};
+    FY[sl] = typename P::facc{};
+    FZ[sl] = typename P::facc{};
 }
 
 template <class P>
@@ -65,9 +65,10 @@ __device__ __forceinline__ void total_force(int a, int sl, const typename P::fac
         f[1] = (double)y * (1.0 / kForceScale);
         f[2] = (double)z * (1.0 / kForceScale);
     } else {
-        f[0] = FAx[a] + FSx[sl];
-        f[1] = FAy[a] + FSy[sl];
-        f[2] = FAz[a] + FSz[sl];
+        // exact integer sums of the two parts, then one conversion
+        f[0] = fix2_value(FAx[a].hi + FSx[sl].hi, FAx[a].lo + FSx[sl].lo);
+        f[1] = fix2_value(FAy[a].hi + FSy[sl].hi, FAy[a].lo + FSy[sl].lo);
+        f[2] = fix2_value(FAz[a].hi + FSz[sl].hi, FAz[a].lo + FSz[sl].lo);
     }
 }
 
@@ -140,9 +141,9 @@ __global__ void integrate_kernel(const SysDev* __restrict__ sd, SysCount* sc, co
         const int sl = slot_of_atom[a];
         double f[3];
         total_force<P>(a, sl, FAx, FAy, FAz, FSx, FSy, FSz, f);
-        FSx[sl] = 0;
-        FSy[sl] = 0;
-        FSz[sl] = 0;
+        FSx[sl] = typename P::facc{};
+        FSy[sl] = typename P::facc{};
+        FSz[sl] = typename P::facc{};
         // a system whose neighbour-list build overflowed lacks pair forces: it is
         // frozen at the last step it completed (the host reports it as failed)
         const bool frozen = sc[s].overflow != 0;

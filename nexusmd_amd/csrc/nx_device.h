@@ -126,11 +126,21 @@ __device__ __forceinline__ void side_priority() { __builtin_amdgcn_s_setprio(2);
 
 // ------------------------------------------------------------ precision
 
+// FP64 force accumulator: v = hi 2^-20 + lo 2^-74, each part summed exactly
+// in a 64-bit integer, so a sum of contributions is the same whatever their
+// order (the reference sums serially and is reproducible; double atomics are
+// not).  Range |v| < 8.7e12, resolution 2^-74 (5e-23) per contribution.
+struct Fix2 {
+    long long hi, lo;
+};
+
 struct PrecFP64 {
     using real = double;
     using real2 = double2;
     using real4 = double4;
-    using facc = double;          // force accumulator element
+    using facc = Fix2;            // force accumulator element
+    using gacc = long long;       // PME charge grid element, 2^40 fixed point
+    static constexpr double grid_scale = 1099511627776.0;
     static constexpr bool mixed = false;
     static constexpr int pair_waves_per_simd = 1;   // minimum pair-kernel waves per SIMD the registers must allow
 };
@@ -140,6 +150,8 @@ struct PrecMixed {
     using real2 = float2;
     using real4 = float4;
     using facc = long long;       // 2^32 fixed point
+    using gacc = long long;       // PME charge grid element, 2^32 fixed point
+    static constexpr double grid_scale = 4294967296.0;
     static constexpr bool mixed = true;
     static constexpr int pair_waves_per_simd = 4;
     // Three pair-kernel waves per SIMD (the persistent grid's three blocks per
@@ -160,7 +172,21 @@ __device__ __forceinline__ long long to_fixed(double v) {
     return __double2ll_rn(s);
 }
 
-__device__ __forceinline__ void facc_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+// the two parts of v: hi = v 2^20 rounded, lo = the rest times 2^74 rounded
+// (v - hi 2^-20 is exact: the low bits of v)
+__device__ __forceinline__ void fix2_parts(double v, long long& hi, long long& lo) {
+    double s = v * 1048576.0;
+    s = fmin(fmax(s, -9.2e18), 9.2e18);
+    const double h = rint(s);
+    hi = (long long)h;
+    lo = (long long)rint((s - h) * 18014398509481984.0);     // (v - hi 2^-20) 2^74 = (s - h) 2^54
+}
+__device__ __forceinline__ void facc_add(Fix2* p, double v) {
+    long long hi, lo;
+    fix2_parts(v, hi, lo);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&p->hi), static_cast<unsigned long long>(hi));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&p->lo), static_cast<unsigned long long>(lo));
+}
 __device__ __forceinline__ void facc_add(long long* p, float v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(to_fixed(v)));
 }
@@ -168,11 +194,24 @@ __device__ __forceinline__ void facc_add(long long* p, double v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(to_fixed(v)));
 }
 // Non-atomic add / store, for a location that no other thread writes meanwhile.
-__device__ __forceinline__ void facc_set_add(double* p, double v) { *p += v; }
+__device__ __forceinline__ void facc_set_add(Fix2* p, double v) {
+    long long hi, lo;
+    fix2_parts(v, hi, lo);
+    p->hi += hi;
+    p->lo += lo;
+}
 __device__ __forceinline__ void facc_set_add(long long* p, float v) { *p += to_fixed(v); }
-__device__ __forceinline__ void facc_set(double* p, double v) { *p = v; }
+__device__ __forceinline__ void facc_set(Fix2* p, double v) {
+    long long hi, lo;
+    fix2_parts(v, hi, lo);
+    p->hi = hi;
+    p->lo = lo;
+}
 __device__ __forceinline__ void facc_set(long long* p, float v) { *p = to_fixed(v); }
-__device__ __forceinline__ double facc_get(const double* p) { return *p; }
+__device__ __forceinline__ double fix2_value(long long hi, long long lo) {
+    return (double)hi * (1.0 / 1048576.0) + (double)lo * (1.0 / 18889465931478580854784.0);   // 2^-20, 2^-74
+}
+__device__ __forceinline__ double facc_get(const Fix2* p) { return fix2_value(p->hi, p->lo); }
 __device__ __forceinline__ double facc_get(const long long* p) { return static_cast<double>(*p) * (1.0 / kForceScale); }
 
 // Energy accumulation in 2^30 fixed point.  Every contribution is rounded to

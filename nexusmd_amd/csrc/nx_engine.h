@@ -119,7 +119,8 @@ struct Batch {
     int n_tors = 0;
 
     // PME
-    facc* rgrid = nullptr;          // spread charges [sum Kx*Ky*Kz]
+    using gacc = typename P::gacc;
+    gacc* rgrid = nullptr;          // spread charges [sum Kx*Ky*Kz], fixed point (P::grid_scale)
     real2* cgrid = nullptr;         // half spectrum [sum Kx*Ky*(Kz/2+1)]
     real* phi = nullptr;            // potential grid [sum Kx*Ky*Kz]
     real* eta = nullptr;            // influence function on the half spectrum

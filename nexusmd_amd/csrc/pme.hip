@@ -69,17 +69,26 @@ __device__ __forceinline__ void grid_index(real x, double invL, int K, int& idx,
     idx = ti >= K ? ti - K : ti;     // t is in [0, K]
 }
 
-__device__ __forceinline__ void grid_add(double* g, double v) { unsafeAtomicAdd(g, v); }
-__device__ __forceinline__ void grid_add(long long* g, float v) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(to_fixed(v)));
+// The charge grid is fixed point in both paths (exact, order-free sums): 2^32
+// in the mixed path (FP32 contributions), 2^40 in the FP64 path (grid values
+// stay far below its range of 8e6, and its resolution, 9e-13, is far below the
+// FP64 path's error bound).
+__device__ __forceinline__ long long grid_fixed(float v) { return to_fixed(v); }
+__device__ __forceinline__ long long grid_fixed(double v) {
+    double s = v * PrecFP64::grid_scale;
+    s = fmin(fmax(s, -9.2e18), 9.2e18);
+    return (long long)rint(s);
 }
-// LDS accumulation: fixed point (mixed, exact and order-free) or FP64
-__device__ __forceinline__ void grid_add_lds(double* g, double v) { atomicAdd(g, v); }
-__device__ __forceinline__ void grid_add_lds(long long* g, float v) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(to_fixed(v)));
+template <typename R>
+__device__ __forceinline__ void grid_add(long long* g, R v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(grid_fixed(v)));
+}
+// LDS accumulation (a ds_add_u64)
+template <typename R>
+__device__ __forceinline__ void grid_add_lds(long long* g, R v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(grid_fixed(v)));
 }
 // adding an already accumulated value
-__device__ __forceinline__ void grid_add_acc(double* g, double v) { unsafeAtomicAdd(g, v); }
 __device__ __forceinline__ void grid_add_acc(long long* g, long long v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(g), static_cast<unsigned long long>(v));
 }
@@ -88,7 +97,7 @@ __device__ __forceinline__ void grid_add_acc(long long* g, long long v) {
 template <class P>
 __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount* __restrict__ sc,
                                   const int* __restrict__ slot_sys, const int* __restrict__ atom_of_slot,
-                                  const typename P::real4* __restrict__ pos, typename P::facc* rgrid,
+                                  const typename P::real4* __restrict__ pos, typename P::gacc* rgrid,
                                   int n_slots_total) {
     side_priority();
     using real = typename P::real;
@@ -113,7 +122,7 @@ __global__ void pme_spread_kernel(const SysDev* __restrict__ sd, const SysCount*
     bspline5<real>(f2, tz, dtz);
     const real qx = p.w * tx[ix];
     const int xi = (i0 + ix) % Kx;
-    typename P::facc* g = rgrid + d.rgrid_off + (long long)xi * Ky * Kz;
+    typename P::gacc* g = rgrid + d.rgrid_off + (long long)xi * Ky * Kz;
 #pragma unroll
     for (int iy = 0; iy < 5; ++iy) {
         const int yi = (i1 + iy) % Ky;
@@ -149,10 +158,10 @@ __global__ __launch_bounds__(256) void pme_spread_tiled(const SysDev* __restrict
                                                         const int* __restrict__ col_slot,
                                                         const float* __restrict__ slot_zkey,
                                                         const typename P::real4* __restrict__ pos,
-                                                        typename P::facc* rgrid) {
+                                                        typename P::gacc* rgrid) {
     side_priority();
     using real = typename P::real;
-    using acc = typename P::facc;
+    using acc = typename P::gacc;
     __shared__ acc g[kSpreadLdsPoints];
     __shared__ int c_first[kSpreadTileCols], c_cnt[kSpreadTileCols], c_a[kSpreadTileCols], c_e[kSpreadTileCols];
     __shared__ int c_pre[kSpreadTileCols + 1];
@@ -510,13 +519,13 @@ __device__ R2* fft_lines_lds(R2* a, R2* b, int N, int lines, int ls, int es, con
     return src;
 }
 
-__device__ __forceinline__ float fixed_to_real(long long v, float) { return (float)((double)v * (1.0 / kForceScale)); }
-__device__ __forceinline__ double fixed_to_real(double v, double) { return v; }
+__device__ __forceinline__ float fixed_to_real(long long v, float) { return (float)((double)v * (1.0 / PrecMixed::grid_scale)); }
+__device__ __forceinline__ double fixed_to_real(long long v, double) { return (double)v * (1.0 / PrecFP64::grid_scale); }
 
 // Z pass forward: R2C.  item = (sys, first line, n lines)
 template <class P>
 __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, const int4* __restrict__ items,
-                                                 typename P::facc* __restrict__ rgrid,
+                                                 typename P::gacc* __restrict__ rgrid,
                                                  typename P::real2* cgrid, const typename P::real2* __restrict__ tw) {
     side_priority();
     using real = typename P::real;
@@ -532,12 +541,12 @@ __global__ __launch_bounds__(256) void fft_z_fwd(const SysDev* __restrict__ sd, 
     R2* TW = B + nl * Kz;
     load_twiddles(TW, tw + d.tw_off[2], Kz);
     // the spread grid is cleared as it is read: the next spread adds into it
-    typename P::facc* g = rgrid + d.rgrid_off + (long long)line0 * Kz;
+    typename P::gacc* g = rgrid + d.rgrid_off + (long long)line0 * Kz;
     for (int i = threadIdx.x; i < nl * Kz; i += blockDim.x) {
         R2 v;
         v.x = fixed_to_real(g[i], (real)0);
         v.y = 0;
-        g[i] = typename P::facc(0);
+        g[i] = typename P::gacc(0);
         A[i] = v;
     }
     __syncthreads();
@@ -722,7 +731,7 @@ __global__ __launch_bounds__(256) void fft_x_plain(const SysDev* __restrict__ sd
 template <class P>
 __global__ __launch_bounds__(kPlaneThreads) void fft_plane_fwd(const SysDev* __restrict__ sd,
                                                               const int4* __restrict__ items,
-                                                              typename P::facc* __restrict__ rgrid,
+                                                              typename P::gacc* __restrict__ rgrid,
                                                               typename P::real2* cgrid,
                                                               const typename P::real2* __restrict__ tw) {
     side_priority();
@@ -742,17 +751,17 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_fwd(const SysDev* __r
     load_twiddles(TWy, tw + d.tw_off[1], Ky);
     load_twiddles(TWz, tw + d.tw_off[2], Kz);
     // the spread grid is cleared as it is read: the next spread adds into it
-    typename P::facc* g = rgrid + d.rgrid_off + (long long)x * Ky * Kz;
+    typename P::gacc* g = rgrid + d.rgrid_off + (long long)x * Ky * Kz;
     const float inv_kz = 1.0f / (float)Kz;
     for (int i = threadIdx.x; i < np * Kz; i += blockDim.x) {
         const int p = qdiv(i, inv_kz), z = i - p * Kz;
         R2 v;
         v.x = fixed_to_real(g[(2 * p) * Kz + z], (real)0);
-        g[(2 * p) * Kz + z] = typename P::facc(0);
+        g[(2 * p) * Kz + z] = typename P::gacc(0);
         v.y = (real)0;
         if (2 * p + 1 < Ky) {
             v.y = fixed_to_real(g[(2 * p + 1) * Kz + z], (real)0);
-            g[(2 * p + 1) * Kz + z] = typename P::facc(0);
+            g[(2 * p + 1) * Kz + z] = typename P::gacc(0);
         }
         A[i] = v;
     }
@@ -1027,14 +1036,14 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
         for (int k = 0; k < nzh; k += T) yi.push_back(make_int4(0, x, k, 0));
     for (int y = 0; y < ny; ++y)
         for (int k = 0; k < nzh; k += T) xi.push_back(make_int4(0, y, k, 0));
-    SysDev* dd; R2* dtw; int4 *dz, *dy, *dx; typename P::facc* rg; R2* cg; real* ph;
+    SysDev* dd; R2* dtw; int4 *dz, *dy, *dx; typename P::gacc* rg; R2* cg; real* ph;
     const size_t G = (size_t)nx * ny * nz, Gh = (size_t)nx * ny * nzh;
     hipMalloc(&dd, sizeof(SysDev));
     hipMalloc(&dtw, tw.size() * sizeof(R2));
     hipMalloc(&dz, zi.size() * sizeof(int4));
     hipMalloc(&dy, yi.size() * sizeof(int4));
     hipMalloc(&dx, xi.size() * sizeof(int4));
-    hipMalloc(&rg, G * sizeof(typename P::facc));
+    hipMalloc(&rg, G * sizeof(typename P::gacc));
     hipMalloc(&cg, Gh * sizeof(R2));
     hipMalloc(&ph, G * sizeof(real));
     hipMemcpyAsync(dd, &d, sizeof(SysDev), hipMemcpyHostToDevice, st);
@@ -1062,12 +1071,11 @@ void fft3d_roundtrip(int nx, int ny, int nz, const double* in, double* out, bool
         hipFuncSetAttribute((const void*)fft_x_block<P, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)xl);
     }
     if (forward) {
-        std::vector<typename P::facc> h(G);
+        std::vector<typename P::gacc> h(G);
         for (size_t i = 0; i < G; ++i) {
-            if constexpr (P::mixed) h[i] = (long long)llrint(in[i] * kForceScale);
-            else h[i] = in[i];
+            h[i] = (long long)llrint(in[i] * P::grid_scale);
         }
-        hipMemcpyAsync(rg, h.data(), G * sizeof(typename P::facc), hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(rg, h.data(), G * sizeof(typename P::gacc), hipMemcpyHostToDevice, st);
         if (plane) {
             fft_plane_fwd<P><<<(unsigned)pi.size(), kPlaneThreads, pl, st>>>(dd, dp, rg, cg, dtw);
             fft_x_block<P, 0><<<(unsigned)bi.size(), kXPassThreads, xl, st>>>(dd, db, cg, nullptr, dtw, nullptr);
@@ -1160,11 +1168,11 @@ __global__ void noalloc_items_kernel(int4* it, int nx, int ny, int nzh, int T, i
 
 // In place: the doubles of the input become 2^32 fixed point (as the host
 // conversion of fft3d_roundtrip: round to nearest, ties to even).
-__global__ void noalloc_to_fixed_kernel(void* g, long long n) {
+__global__ void noalloc_to_fixed_kernel(void* g, long long n, double scale) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double v = reinterpret_cast<const double*>(g)[i];
-    reinterpret_cast<long long*>(g)[i] = __double2ll_rn(v * kForceScale);
+    reinterpret_cast<long long*>(g)[i] = __double2ll_rn(v * scale);
 }
 
 }  // namespace
@@ -1218,7 +1226,7 @@ nx_status fft3d_r2c_noalloc(int device, int nx, int ny, int nz, const double* in
     SysDev* dd = reinterpret_cast<SysDev*>(base);
     R2* dtw = reinterpret_cast<R2*>(base + o_tw);
     int4* items = reinterpret_cast<int4*>(base + o_it);
-    typename P::facc* rg = reinterpret_cast<typename P::facc*>(base + o_rg);
+    typename P::gacc* rg = reinterpret_cast<typename P::gacc*>(base + o_rg);
     R2* cg = reinterpret_cast<R2*>(base + o_cg);
     const hipStream_t st = nullptr;
     // the per-call tables, from host values on the stack
@@ -1239,7 +1247,7 @@ nx_status fft3d_r2c_noalloc(int device, int nx, int ny, int nz, const double* in
     noalloc_items_kernel<<<(n0 + n1 + n2 + 255) / 256, 256, 0, st>>>(items, nx, ny, nzh, T, plane ? 1 : 0, n0, n1, n2);
     if ((e = hipMemcpyAsync(rg, in, (size_t)G * sizeof(double), hipMemcpyHostToDevice, st)) != hipSuccess)
         return fail(e, "copy");
-    if constexpr (P::mixed) noalloc_to_fixed_kernel<<<(unsigned)((G + 255) / 256), 256, 0, st>>>(rg, G);
+    noalloc_to_fixed_kernel<<<(unsigned)((G + 255) / 256), 256, 0, st>>>(rg, G, P::grid_scale);
     if (plane) {
         const size_t pl = plane_lds_bytes<R2>(ny, nz), xl = xpass_lds_bytes<R2>(nx, nzh);
         if ((e = hipFuncSetAttribute((const void*)fft_plane_fwd<P>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -161,8 +161,8 @@ def test_repeated_evaluations_agree(prec):
     """The pair kernel hands its chunks out at run time, and the forward FFT
     clears the spread grid for the next call.  Evaluating the same positions
     again, in the same context and in a second one, gives the same forces and
-    energies: bit for bit in the mixed path (fixed-point sums), to rounding in
-    the FP64 path (floating-point atomics)."""
+    energies bit for bit in both paths (every sum that crosses wavefronts is
+    fixed point: 2^32 in the mixed path, two 64-bit parts in the FP64 path)."""
     sys = _jiggle(S.water_box(n_mol=8000, L=6.2, grid=(64, 64, 64), seed=21), 3)
     a = _ctx(sys, prec, skin=0.1)
     b = _ctx(sys, prec, skin=0.1)
@@ -170,19 +170,15 @@ def test_repeated_evaluations_agree(prec):
     f2, e2 = a.compute_forces(L.ALL)
     f3, e3 = b.compute_forces(L.ALL)
     for f, e in ((f2, e2), (f3, e3)):
-        if prec == L.MIXED:
-            assert np.array_equal(f, f1)
-            assert all(e[k] == e1[k] for k in e1)
-        else:
-            assert rel_rms(f, f1) <= 1e-12
-            assert all(abs(e[k] - e1[k]) <= 1e-9 * max(1.0, abs(e1[k])) for k in e1)
+        assert np.array_equal(f, f1)
+        assert all(e[k] == e1[k] for k in e1)
     a.close()
     b.close()
 
 
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
 def test_graph_steps_match_plain(chain, prec):
-    """Graph-replayed steps equal step-by-step launches (mixed: bitwise)."""
+    """Graph-replayed steps equal step-by-step launches, bit for bit."""
     a = _ctx(chain, prec, skin=0.1)
     b = _ctx(chain, prec, skin=0.1)
     a.step(20, 0.001)               # graph path
@@ -190,10 +186,7 @@ def test_graph_steps_match_plain(chain, prec):
         b.step(1, 0.001)            # plain launches
     pa, va = a.get_state()
     pb, vb = b.get_state()
-    if prec == L.MIXED:
-        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
-    else:
-        assert np.allclose(pa, pb, rtol=0, atol=1e-9)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
     a.close()
     b.close()
 
@@ -236,7 +229,7 @@ def test_switch_forces_vs_oracle(prec):
 def test_timed_graph_steps_match_untimed(chain, prec):
     """Steps replayed with the pair kernel bracketed by events (the pair-kernel
     sampling mode of the benchmark: every step split into graphs around the
-    pair kernel) end in the state of plain graph replays (mixed: bitwise); the
+    pair kernel) end in the state of plain graph replays, bit for bit; the
     step and pair events cover every step once."""
     a = _ctx(chain, prec, skin=0.1)
     b = _ctx(chain, prec, skin=0.1)
@@ -253,10 +246,7 @@ def test_timed_graph_steps_match_untimed(chain, prec):
     b.enable_timing(L.TIMING_OFF)
     pa, va = a.get_state()
     pb, vb = b.get_state()
-    if prec == L.MIXED:
-        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
-    else:
-        assert np.allclose(pa, pb, rtol=0, atol=1e-9)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
     a.close()
     b.close()
 

@@ -1,11 +1,11 @@
 """Continuous-batching server (nx_server) on the GPU.
 
 The property under test: a request's result does not depend on what else is
-on the GPU.  In the mixed path all force and energy sums are 64-bit fixed
-point and each system's neighbour list is rebuilt from the same positions
-whether it runs alone or in a changing batch, so alone and in the mix agree
-bit for bit.  The FP64 path sums doubles with hardware atomics in an order
-that varies from run to run, so it is held to a relative tolerance instead.
+on the GPU.  All force, grid and energy sums that cross wavefronts are fixed
+point (64-bit in the mixed path, two 64-bit parts for FP64 forces), and each
+system's neighbour list is rebuilt from the same positions whether it runs
+alone or in a changing batch, so alone and in the mix agree bit for bit in
+both paths.
 """
 import numpy as np
 import pytest
@@ -14,7 +14,6 @@ from nexusmd_amd import _lib as L
 from nexusmd_amd import engine as E
 from nexusmd_amd import systems as S
 
-from _util import rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -70,29 +69,30 @@ def test_mixed_alone_equals_mix_bitwise(requests):
         assert e["potential"] == ea["potential"]
 
 
-def test_fp64_alone_vs_mix_within_tolerance(requests):
+def test_fp64_alone_equals_mix_bitwise(requests):
     reqs = requests[:4]
     print("[gpu] serving fp64: sizes %s" % [s.n for s, _ in reqs], flush=True)
     mix, _ = _mix(reqs, L.FP64)
     for (s, k), (p, v, e) in zip(reqs, mix):
         pa, va, ea = _alone(s, k, L.FP64)
-        assert rel_rms(p, pa) <= 1e-9
-        assert rel_rms(v, va) <= 1e-5
-        assert abs(e["potential"] - ea["potential"]) <= 1e-6 * abs(ea["potential"])
+        assert np.array_equal(p, pa)
+        assert np.array_equal(v, va)
+        assert e["potential"] == ea["potential"]
 
 
 def test_fp64_repeated_runs_agree(requests):
     """A request run again and again, with other servers allocating and freeing
-    device memory in between, gives the same FP64 result each time (new device
-    buffers are cleared in stream order before their uploads)."""
+    device memory in between, gives the same FP64 result each time, bit for bit
+    (new device buffers are cleared in stream order before their uploads, and
+    the FP64 sums are fixed point)."""
     s, k = requests[3]
     print("[gpu] serving fp64 repeats: n=%d steps=%d" % (s.n, k), flush=True)
     p0, v0, e0 = _alone(s, k, L.FP64)
     for _ in range(8):
         _mix(requests[:3], L.MIXED)
         p, v, e = _alone(s, k, L.FP64)
-        assert rel_rms(p, p0) <= 1e-9
-        assert rel_rms(v, v0) <= 1e-5
+        assert np.array_equal(p, p0)
+        assert np.array_equal(v, v0)
 
 
 def test_admission_by_memory_budget(requests):
