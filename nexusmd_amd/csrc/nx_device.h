@@ -139,8 +139,8 @@ struct PrecFP64 {
     using real2 = double2;
     using real4 = double4;
     using facc = Fix2;            // force accumulator element
-    using gacc = long long;       // PME charge grid element, 2^40 fixed point
-    static constexpr double grid_scale = 1099511627776.0;
+    using gacc = long long;       // PME charge grid element, 2^48 fixed point
+    static constexpr double grid_scale = 281474976710656.0;
     static constexpr bool mixed = false;
     static constexpr int pair_waves_per_simd = 1;   // minimum pair-kernel waves per SIMD the registers must allow
 };
