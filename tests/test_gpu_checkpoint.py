@@ -1,7 +1,7 @@
 """Checkpoint and restart: N steps, a checkpoint, a restart and M more steps
-end where N + M uninterrupted steps end -- bit for bit in the mixed path
-(fixed-point force sums over the same neighbour list), to 1e-9 in the FP64
-path (floating-point atomics).  The reference has no restart (v(0) = 0,
+end where N + M uninterrupted steps end -- bit for bit in both paths
+(fixed-point force sums over the same neighbour list).  The reference has no
+restart (v(0) = 0,
 Initializer.cpp:21; box from the positions, PeriodicBoundaryCondition.cpp:
 11-88), so these are properties of the extension, not parity checks.
 
@@ -22,7 +22,7 @@ from nexusmd_amd import engine as E
 from nexusmd_amd import systems as S
 from nexusmd_amd import xmlio
 
-from _util import golden, rel_rms, system_from
+from _util import golden, system_from
 
 pytestmark = pytest.mark.gpu
 
@@ -36,10 +36,7 @@ def water():
 
 
 def _same(prec, a, b):
-    if prec == L.MIXED:
-        np.testing.assert_array_equal(a, b)
-    else:
-        assert rel_rms(a, b) <= 1e-9
+    np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
@@ -108,9 +105,8 @@ def _state_arrays(path):
 def test_simulate_checkpoint_restart(nx, tmp_path, choice):
     """Simulate 12 steps with a checkpoint at the end, against 7 steps, a
     checkpoint, and a restart for 5 more (reports every 3 steps): the final
-    state files agree (mixed: the same text), and the restart appended its
-    reports so that the report files equal the uninterrupted run's (mixed:
-    byte for byte)."""
+    state files are the same text, and the restart appended its reports so
+    that the report files equal the uninterrupted run's byte for byte."""
     c0 = golden("chain1500")
     s = system_from(c0, "chain1500")
     full, half = tmp_path / "full", tmp_path / "half"
@@ -127,15 +123,12 @@ def test_simulate_checkpoint_restart(nx, tmp_path, choice):
     pf, vf, bf = _state_arrays(full / "end.xml")
     ph, vh, bh = _state_arrays(half / "end.xml")
     assert pf.shape == (s.n, 3) and bf.shape == (s.n, 3)
-    if choice == "GPU_MIXED":
-        assert open(full / "end.xml").read() == open(half / "end.xml").read()
-        for name in ("TotalEnergyout.pdb", "PVFReportout.pdb"):
-            assert open(full / name, "rb").read() == open(half / name, "rb").read(), name
-        import re
-        strip = lambda t: re.sub(rb"Nexaus 1\.0\.0, \d{4}-\d{2}-\d{2}", b"", t)
-        assert strip(open(full / "out.pdb", "rb").read()) == strip(open(half / "out.pdb", "rb").read())
-    else:
-        assert rel_rms(ph, pf) <= 1e-9 and rel_rms(vh, vf) <= 1e-7
+    assert open(full / "end.xml").read() == open(half / "end.xml").read()
+    for name in ("TotalEnergyout.pdb", "PVFReportout.pdb"):
+        assert open(full / name, "rb").read() == open(half / name, "rb").read(), name
+    import re
+    strip = lambda t: re.sub(rb"Nexaus 1\.0\.0, \d{4}-\d{2}-\d{2}", b"", t)
+    assert strip(open(full / "out.pdb", "rb").read()) == strip(open(half / "out.pdb", "rb").read())
     rows = [l.split()[0] for l in open(half / "TotalEnergyout.pdb").read().splitlines()[1:]]
     assert rows == ["0", "2", "5", "8", "11"]
 
