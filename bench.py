@@ -745,7 +745,11 @@ def main():
         "roofline": roofline,
         "pair_timing": pair_m,
         "kernels_mixed": kernels,
-        "pair_list": {"entries": stats_m["list_entries"], "tiles": tiles, "rebuilds_in_timed_region": stats_m["rebuilds"]},
+        "pair_list": {"entries": stats_m["list_entries"], "tiles": tiles, "rebuilds_in_timed_region": stats_m["rebuilds"],
+                      # over the context's life (warm-up and timed steps): steps run as the
+                      # graph without the list kernels, and steps run again after one froze
+                      "steps_without_list_kernels": stats_m.get("steps_without_list"),
+                      "steps_repeated": stats_m.get("steps_repeated")},
     }
     if not args.no_cpu:
         cb = cpu_baseline(sysm, args.cpu_steps)
